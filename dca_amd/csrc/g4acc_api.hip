@@ -1,0 +1,682 @@
+// C ABI of the g4acc library: handle life cycle, argument checking, host/device staging of the
+// walkers' inputs and the launch sequence of one batch. Nothing here throws across the ABI.
+#include "../../include/g4acc.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "g4acc_common.hpp"
+#include "g4acc_kernels.hpp"
+
+using namespace g4acc;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+size_t workspace_budget_bytes() { return size_t(8) << 30; }
+
+}  // namespace
+
+struct g4acc_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Geo g{};
+  int precision = G4ACC_FP64;
+  int sum_mode = G4ACC_SUM_FP64;
+  int max_walkers = 1, max_vertices = 1, chunk = 1;
+  std::vector<int> channels, w_ex, k_ex;
+  size_t g4_entries = 0;   // per channel
+  bool initialized = false;
+  bool g4_present = true;
+  std::string err;
+
+  // device memory
+  std::vector<void*> allocations;
+  size_t bytes = 0;
+  int *d_kadd = nullptr, *d_ksub = nullptr, *d_kex = nullptr, *d_wex = nullptr;
+  void* d_dft = nullptr;           // Cplx<Real>[Nc*Nc]
+  double2* d_bphase = nullptr;     // [nb*Nc] or null
+  double2* d_G0 = nullptr;
+  std::vector<void*> d_g4;         // per channel, 16 B per entry
+  int *d_perm = nullptr, *d_off = nullptr, *d_err = nullptr;
+  void *d_T = nullptr, *d_B = nullptr, *d_A = nullptr, *d_E = nullptr, *d_G = nullptr;
+  SlotDesc* d_desc = nullptr;
+  SlotDesc* h_desc = nullptr;      // pinned
+  double* d_stage_f = nullptr;     // device staging of host inputs (M, tau)
+  int* d_stage_i = nullptr;        // (site, band)
+  double* h_stage_f = nullptr;     // pinned
+  int* h_stage_i = nullptr;
+  size_t stage_f_cap = 0, stage_i_cap = 0;
+  hipEvent_t staged = nullptr;     // staging buffers free once this event completes
+  bool staged_pending = false;
+  // stage timing (profiling hook)
+  bool timing = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  double t_mtog_ms = 0, t_g4_ms = 0;
+  long long timed_batches = 0;
+
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+  int check(hipError_t e, const char* what) {
+    if (e == hipSuccess)
+      return G4ACC_OK;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? G4ACC_OUT_OF_MEMORY : G4ACC_DEVICE_ERROR;
+  }
+  template <typename T>
+  int alloc(T** p, size_t count, const char* what) {
+    void* q = nullptr;
+    const size_t b = std::max<size_t>(count * sizeof(T), 16);
+    hipError_t e = hipMalloc(&q, b);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      err = std::string("hipMalloc(") + what + ", " + std::to_string(b) + " B) failed: " +
+            hipGetErrorString(e);
+      return e == hipErrorOutOfMemory ? G4ACC_OUT_OF_MEMORY : G4ACC_DEVICE_ERROR;
+    }
+    allocations.push_back(q);
+    bytes += b;
+    *p = static_cast<T*>(q);
+    return G4ACC_OK;
+  }
+  size_t cplx_bytes() const { return precision == G4ACC_FP64 ? 16 : 8; }
+};
+
+#define G4_TRY(expr)         \
+  do {                       \
+    const int _rc = (expr);  \
+    if (_rc != G4ACC_OK)     \
+      return _rc;            \
+  } while (0)
+
+namespace {
+
+int validate_geometry(const g4acc_geometry* gm, std::string& why) {
+  if (!gm) {
+    why = "geometry is NULL";
+    return G4ACC_INVALID_ARG;
+  }
+  if (gm->n_bands < 1 || gm->n_bands > 4) {
+    why = "n_bands must be in [1, 4]";
+    return G4ACC_INVALID_ARG;
+  }
+  if (gm->n_sites < 1 || gm->n_freq < 2 || gm->n_freq % 2 || !(gm->beta > 0)) {
+    why = "n_sites >= 1, even n_freq >= 2 and beta > 0 are required";
+    return G4ACC_INVALID_ARG;
+  }
+  if (gm->n_w_ex < 1 || !gm->w_ex || gm->n_k_ex < 1 || !gm->k_ex || gm->n_channels < 1 ||
+      !gm->channels || !gm->k_add || !gm->k_sub || !gm->dft || !gm->g0) {
+    why = "missing exchange lists, channels, k tables, dft or g0";
+    return G4ACC_INVALID_ARG;
+  }
+  for (int i = 0; i < gm->n_k_ex; ++i)
+    if (gm->k_ex[i] < 0 || gm->k_ex[i] >= gm->n_sites) {
+      why = "k_ex index out of range";
+      return G4ACC_INVALID_ARG;
+    }
+  const int nc2 = gm->n_sites * gm->n_sites;
+  for (int i = 0; i < nc2; ++i)
+    if (gm->k_add[i] < 0 || gm->k_add[i] >= gm->n_sites || gm->k_sub[i] < 0 ||
+        gm->k_sub[i] >= gm->n_sites) {
+      why = "k_add / k_sub entry out of range";
+      return G4ACC_INVALID_ARG;
+    }
+  for (int i = 0; i < gm->n_channels; ++i) {
+    const int c = gm->channels[i];
+    if (c < G4ACC_PARTICLE_HOLE_TRANSVERSE || c > G4ACC_PARTICLE_PARTICLE_UP_DOWN) {
+      why = "Specified four point type not implemented.";
+      return G4ACC_UNSUPPORTED_CHANNEL;
+    }
+    if (i > 0 && gm->channels[i - 1] >= c) {
+      why = "channels must be sorted and unique";
+      return G4ACC_INVALID_ARG;
+    }
+  }
+  return G4ACC_OK;
+}
+
+template <typename Real>
+void upload_dft(std::vector<unsigned char>& buf, const double* dft, int nc) {
+  using C = Cplx<Real>;
+  buf.resize(sizeof(C) * nc * nc);
+  C* p = reinterpret_cast<C*>(buf.data());
+  for (int i = 0; i < nc * nc; ++i)
+    p[i] = C{static_cast<Real>(dft[2 * i]), static_cast<Real>(dft[2 * i + 1])};
+}
+
+// Resolves the walkers of one chunk into slot descriptors; host inputs are packed into the
+// pinned staging buffer and copied once. Returns the number of slots.
+int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int memory) {
+  const int nb = h->g.nb, nc = h->g.nc;
+  // The pinned descriptor and staging buffers are reused: wait until the previous copies are done.
+  if (h->staged_pending) {
+    G4_TRY(h->check(hipEventSynchronize(h->staged), "hipEventSynchronize"));
+    h->staged_pending = false;
+  }
+  size_t of = 0, oi = 0;
+  for (int w = 0; w < nw_; ++w) {
+    const g4acc_walker& wk = *ws[w];
+    for (int s = 0; s < 2; ++s) {
+      SlotDesc& d = h->h_desc[2 * w + s];
+      const int n = wk.n[s];
+      d.n = n;
+      d.sign = wk.sign;
+      if (n == 0) {
+        d.m = nullptr;
+        d.tau = nullptr;
+        d.site = nullptr;
+        d.band = nullptr;
+        d.ld = 1;
+        continue;
+      }
+      if (memory == G4ACC_HOST) {
+        double* fm = h->h_stage_f + of;
+        for (int j = 0; j < n; ++j)
+          std::memcpy(fm + static_cast<size_t>(n) * j, wk.m[s] + static_cast<size_t>(wk.ld[s]) * j,
+                      sizeof(double) * n);
+        std::memcpy(fm + static_cast<size_t>(n) * n, wk.tau[s], sizeof(double) * n);
+        int* im = h->h_stage_i + oi;
+        for (int i = 0; i < n; ++i) {
+          const int r = wk.site[s][i], b = wk.band[s][i];
+          if (r < 0 || r >= nc || b < 0 || b >= nb)
+            return h->fail(G4ACC_INVALID_ARG, "vertex site or band index out of range");
+          im[i] = r;
+          im[n + i] = b;
+        }
+        d.m = h->d_stage_f + of;
+        d.tau = h->d_stage_f + of + static_cast<size_t>(n) * n;
+        d.site = h->d_stage_i + oi;
+        d.band = h->d_stage_i + oi + n;
+        d.ld = n;
+        of += static_cast<size_t>(n) * n + n;
+        oi += 2 * static_cast<size_t>(n);
+      }
+      else {
+        d.m = wk.m[s];
+        d.tau = wk.tau[s];
+        d.site = wk.site[s];
+        d.band = wk.band[s];
+        d.ld = wk.ld[s];
+      }
+    }
+  }
+  const int n_slots = 2 * nw_;
+  if (memory == G4ACC_HOST && of) {
+    G4_TRY(h->check(hipMemcpyAsync(h->d_stage_f, h->h_stage_f, of * sizeof(double),
+                                   hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync(M)"));
+    G4_TRY(h->check(hipMemcpyAsync(h->d_stage_i, h->h_stage_i, oi * sizeof(int),
+                                   hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync(config)"));
+  }
+  G4_TRY(h->check(hipMemcpyAsync(h->d_desc, h->h_desc, sizeof(SlotDesc) * n_slots,
+                                 hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync(desc)"));
+  G4_TRY(h->check(hipEventRecord(h->staged, h->stream), "hipEventRecord"));
+  h->staged_pending = true;
+  return G4ACC_OK;
+}
+
+template <typename Real>
+MtoGArgs<Real> mtog_args(g4acc_handle* h, int n_slots) {
+  MtoGArgs<Real> a;
+  a.g = h->g;
+  a.desc = h->d_desc;
+  a.n_slots = n_slots;
+  a.n_max = h->max_vertices;
+  a.chunk = h->chunk;
+  a.perm = h->d_perm;
+  a.off = h->d_off;
+  a.T = static_cast<Cplx<Real>*>(h->d_T);
+  a.B = static_cast<Cplx<Real>*>(h->d_B);
+  a.A = static_cast<Cplx<Real>*>(h->d_A);
+  a.E = static_cast<Cplx<Real>*>(h->d_E);
+  a.G = static_cast<Cplx<Real>*>(h->d_G);
+  a.dft = static_cast<const Cplx<Real>*>(h->d_dft);
+  a.bphase = h->d_bphase;
+  a.G0 = h->d_G0;
+  return a;
+}
+
+template <typename Real>
+void run_mtog(g4acc_handle* h, int n_slots) {
+  launch_sort(h->d_desc, h->g, n_slots, h->max_vertices, h->d_perm, h->d_off, h->d_err, h->stream);
+  launch_mtog<Real>(mtog_args<Real>(h, n_slots), h->stream);
+}
+
+template <typename Real>
+void run_g4(g4acc_handle* h, int n_walkers) {
+  for (size_t c = 0; c < h->channels.size(); ++c) {
+    G4Args<Real> a;
+    a.g = h->g;
+    a.channel = h->channels[c];
+    a.G = static_cast<const Cplx<Real>*>(h->d_G);
+    a.desc = h->d_desc;
+    a.n_walkers = n_walkers;
+    a.k_add = h->d_kadd;
+    a.k_sub = h->d_ksub;
+    a.k_ex = h->d_kex;
+    a.w_ex = h->d_wex;
+    a.sum_mode = h->sum_mode;
+    a.g4 = h->d_g4[c];
+    launch_g4<Real>(a, h->stream);
+  }
+}
+
+int run_batch(g4acc_handle* h, int n_walkers) {
+  const int n_slots = 2 * n_walkers;
+  if (h->timing)
+    G4_TRY(h->check(hipEventRecord(h->ev[0], h->stream), "hipEventRecord"));
+  if (h->precision == G4ACC_FP64)
+    run_mtog<double>(h, n_slots);
+  else
+    run_mtog<float>(h, n_slots);
+  if (h->timing)
+    G4_TRY(h->check(hipEventRecord(h->ev[1], h->stream), "hipEventRecord"));
+  if (h->precision == G4ACC_FP64)
+    run_g4<double>(h, n_walkers);
+  else
+    run_g4<float>(h, n_walkers);
+  if (h->timing) {
+    G4_TRY(h->check(hipEventRecord(h->ev[2], h->stream), "hipEventRecord"));
+    G4_TRY(h->check(hipEventSynchronize(h->ev[2]), "hipEventSynchronize"));
+    float a = 0, b = 0;
+    G4_TRY(h->check(hipEventElapsedTime(&a, h->ev[0], h->ev[1]), "hipEventElapsedTime"));
+    G4_TRY(h->check(hipEventElapsedTime(&b, h->ev[1], h->ev[2]), "hipEventElapsedTime"));
+    h->t_mtog_ms += a;
+    h->t_g4_ms += b;
+    ++h->timed_batches;
+  }
+  return h->check(hipGetLastError(), "kernel launch");
+}
+
+int check_walker(g4acc_handle* h, const g4acc_walker& w) {
+  for (int s = 0; s < 2; ++s) {
+    if (w.n[s] < 0 || w.n[s] > h->max_vertices)
+      return h->fail(G4ACC_INVALID_ARG, "vertex count " + std::to_string(w.n[s]) +
+                                            " outside [0, max_vertices=" +
+                                            std::to_string(h->max_vertices) + "]");
+    if (w.n[s] > 0 && (!w.m[s] || !w.tau[s] || !w.site[s] || !w.band[s] || w.ld[s] < w.n[s]))
+      return h->fail(G4ACC_INVALID_ARG, "walker arrays missing or ld < n");
+  }
+  if (!(w.sign == 1.0 || w.sign == -1.0))
+    return h->fail(G4ACC_INVALID_ARG, "sign must be +1 or -1");
+  return G4ACC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handle** out) {
+  g_create_error.clear();
+  if (!out) {
+    g_create_error = "out is NULL";
+    return G4ACC_INVALID_ARG;
+  }
+  *out = nullptr;
+  std::string why;
+  int rc = validate_geometry(gm, why);
+  if (rc != G4ACC_OK) {
+    g_create_error = why;
+    return rc;
+  }
+  if (!opt || opt->max_walkers < 1 || opt->max_vertices < 1 ||
+      (opt->precision != G4ACC_FP64 && opt->precision != G4ACC_FP32) ||
+      (opt->sum_mode != G4ACC_SUM_FP64 && opt->sum_mode != G4ACC_SUM_FP32_KAHAN)) {
+    g_create_error = "invalid options";
+    return G4ACC_INVALID_ARG;
+  }
+  int n_dev = 0;
+  if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev < 1) {
+    (void)hipGetLastError();
+    g_create_error = "no HIP device available";
+    return G4ACC_DEVICE_ERROR;
+  }
+  if (opt->device < 0 || opt->device >= n_dev) {
+    g_create_error = "device ordinal out of range";
+    return G4ACC_INVALID_ARG;
+  }
+  auto* h = new (std::nothrow) g4acc_handle();
+  if (!h) {
+    g_create_error = "host allocation failed";
+    return G4ACC_OUT_OF_MEMORY;
+  }
+  auto bail = [&](int code) {
+    g_create_error = h->err;
+    g4acc_destroy(h);
+    return code;
+  };
+  h->device = opt->device;
+  if ((rc = h->check(hipSetDevice(h->device), "hipSetDevice")) != G4ACC_OK)
+    return bail(rc);
+  if ((rc = h->check(hipStreamCreate(&h->stream), "hipStreamCreate")) != G4ACC_OK)
+    return bail(rc);
+  if ((rc = h->check(hipEventCreateWithFlags(&h->staged, hipEventDisableTiming), "hipEventCreate")) != G4ACC_OK)
+    return bail(rc);
+  for (auto& e : h->ev)
+    if ((rc = h->check(hipEventCreate(&e), "hipEventCreate")) != G4ACC_OK)
+      return bail(rc);
+
+  Geo& g = h->g;
+  g.nb = gm->n_bands;
+  g.nc = gm->n_sites;
+  g.nw = gm->n_freq;
+  g.beta = gm->beta;
+  g.nq = gm->n_k_ex;
+  g.nv = gm->n_w_ex;
+  g.ext = 0;
+  for (int i = 0; i < gm->n_w_ex; ++i)
+    g.ext = std::max(g.ext, std::abs(gm->w_ex[i]));
+  g.nwE = g.nw + 2 * g.ext;
+  g.n_orb = g.nb * g.nc;
+  g.D = g.n_orb * g.nwE;
+  h->precision = opt->precision;
+  h->sum_mode = opt->sum_mode;
+  h->max_walkers = opt->max_walkers;
+  h->max_vertices = opt->max_vertices;
+  h->channels.assign(gm->channels, gm->channels + gm->n_channels);
+  h->w_ex.assign(gm->w_ex, gm->w_ex + gm->n_w_ex);
+  h->k_ex.assign(gm->k_ex, gm->k_ex + gm->n_k_ex);
+  h->g4_entries = static_cast<size_t>(g.nb) * g.nb * g.nb * g.nb * g.nc * g.nw * g.nc * g.nw *
+                  g.nq * g.nv;
+
+  const int nc2 = g.nc * g.nc;
+  const size_t D = g.D, W2 = 2 * static_cast<size_t>(h->max_walkers), NM = h->max_vertices;
+  const size_t cb = h->cplx_bytes();
+  // Slots whose S1..S4 workspaces (B, A: D x N; E: D x D) exist at once.
+  const size_t per_slot = cb * (2 * D * NM + D * D);
+  h->chunk = static_cast<int>(std::max<size_t>(1, std::min<size_t>(W2, workspace_budget_bytes() / per_slot)));
+  const size_t CH = h->chunk;
+
+  if ((rc = h->alloc(&h->d_kadd, nc2, "k_add")) || (rc = h->alloc(&h->d_ksub, nc2, "k_sub")) ||
+      (rc = h->alloc(&h->d_kex, g.nq, "k_ex")) || (rc = h->alloc(&h->d_wex, g.nv, "w_ex")) ||
+      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_dft), cb * nc2, "dft")) ||
+      (rc = h->alloc(&h->d_G0, static_cast<size_t>(g.nb) * g.nb * 2 * g.nc * g.nwE, "G0")) ||
+      (rc = h->alloc(&h->d_perm, W2 * NM, "perm")) ||
+      (rc = h->alloc(&h->d_off, W2 * (g.n_orb + 1), "off")) || (rc = h->alloc(&h->d_err, 1, "err")) ||
+      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_T), cb * W2 * NM * g.nwE, "T")) ||
+      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_B), cb * CH * NM * D, "B")) ||
+      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_A), cb * CH * NM * D, "A")) ||
+      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_E), cb * CH * D * D, "E")) ||
+      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_G), cb * W2 * D * D, "G")) ||
+      (rc = h->alloc(&h->d_desc, W2, "desc")))
+    return bail(rc);
+  if (gm->band_phase && (rc = h->alloc(&h->d_bphase, static_cast<size_t>(g.nb) * g.nc, "band_phase")))
+    return bail(rc);
+  h->d_g4.assign(h->channels.size(), nullptr);
+  for (auto& p : h->d_g4)
+    if ((rc = h->alloc(reinterpret_cast<double2**>(&p), h->g4_entries, "G4")))
+      return bail(rc);
+  h->stage_f_cap = W2 * (NM * NM + NM);
+  h->stage_i_cap = W2 * 2 * NM;
+  if ((rc = h->alloc(&h->d_stage_f, h->stage_f_cap, "stage_f")) ||
+      (rc = h->alloc(&h->d_stage_i, h->stage_i_cap, "stage_i")))
+    return bail(rc);
+  if ((rc = h->check(hipHostMalloc(reinterpret_cast<void**>(&h->h_desc), sizeof(SlotDesc) * W2),
+                     "hipHostMalloc")) ||
+      (rc = h->check(hipHostMalloc(reinterpret_cast<void**>(&h->h_stage_f), sizeof(double) * h->stage_f_cap),
+                     "hipHostMalloc")) ||
+      (rc = h->check(hipHostMalloc(reinterpret_cast<void**>(&h->h_stage_i), sizeof(int) * h->stage_i_cap),
+                     "hipHostMalloc")))
+    return bail(rc);
+
+  // Constant tables.
+  std::vector<unsigned char> dftbuf;
+  if (h->precision == G4ACC_FP64)
+    upload_dft<double>(dftbuf, gm->dft, g.nc);
+  else
+    upload_dft<float>(dftbuf, gm->dft, g.nc);
+  if ((rc = h->check(hipMemcpy(h->d_kadd, gm->k_add, sizeof(int) * nc2, hipMemcpyHostToDevice), "copy")) ||
+      (rc = h->check(hipMemcpy(h->d_ksub, gm->k_sub, sizeof(int) * nc2, hipMemcpyHostToDevice), "copy")) ||
+      (rc = h->check(hipMemcpy(h->d_kex, gm->k_ex, sizeof(int) * g.nq, hipMemcpyHostToDevice), "copy")) ||
+      (rc = h->check(hipMemcpy(h->d_wex, gm->w_ex, sizeof(int) * g.nv, hipMemcpyHostToDevice), "copy")) ||
+      (rc = h->check(hipMemcpy(h->d_dft, dftbuf.data(), dftbuf.size(), hipMemcpyHostToDevice), "copy")) ||
+      (rc = h->check(hipMemcpy(h->d_G0, gm->g0,
+                               sizeof(double2) * g.nb * g.nb * 2 * g.nc * g.nwE, hipMemcpyHostToDevice),
+                     "copy")) ||
+      (rc = h->check(hipMemset(h->d_err, 0, sizeof(int)), "memset")))
+    return bail(rc);
+  if (gm->band_phase &&
+      (rc = h->check(hipMemcpy(h->d_bphase, gm->band_phase, sizeof(double2) * g.nb * g.nc,
+                               hipMemcpyHostToDevice), "copy")))
+    return bail(rc);
+  // Until resetAccumulation, accumulate is refused (tp_accumulator_gpu.hpp:329-330); G4 starts at 0.
+  for (auto* p : h->d_g4)
+    launch_zero16(p, h->g4_entries, h->stream);
+  if ((rc = h->check(hipStreamSynchronize(h->stream), "hipStreamSynchronize")))
+    return bail(rc);
+  *out = h;
+  return G4ACC_OK;
+}
+
+int g4acc_destroy(g4acc_handle* h) {
+  if (!h)
+    return G4ACC_OK;
+  if (h->stream)
+    (void)hipStreamSynchronize(h->stream);
+  for (void* p : h->allocations)
+    (void)hipFree(p);
+  if (h->h_desc)
+    (void)hipHostFree(h->h_desc);
+  if (h->h_stage_f)
+    (void)hipHostFree(h->h_stage_f);
+  if (h->h_stage_i)
+    (void)hipHostFree(h->h_stage_i);
+  if (h->staged)
+    (void)hipEventDestroy(h->staged);
+  for (auto e : h->ev)
+    if (e)
+      (void)hipEventDestroy(e);
+  if (h->stream)
+    (void)hipStreamDestroy(h->stream);
+  delete h;
+  return G4ACC_OK;
+}
+
+int g4acc_reset(g4acc_handle* h) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  for (auto* p : h->d_g4)
+    launch_zero16(p, h->g4_entries, h->stream);
+  h->initialized = true;
+  h->g4_present = true;
+  return h->check(hipGetLastError(), "zero G4");
+}
+
+int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_walkers, int memory) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  if (!h->initialized)
+    return h->fail(G4ACC_NOT_INITIALIZED, "accumulate called before resetAccumulation");
+  if (n_walkers < 0 || (n_walkers > 0 && !walkers) || (memory != G4ACC_HOST && memory != G4ACC_DEVICE))
+    return h->fail(G4ACC_INVALID_ARG, "invalid batch arguments");
+  if (!h->g4_present)
+    return G4ACC_OK;  // after sumTo the reference's G4 vector is empty: nothing is accumulated
+  G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  std::vector<const g4acc_walker*> todo;
+  todo.reserve(n_walkers);
+  for (int i = 0; i < n_walkers; ++i) {
+    G4_TRY(check_walker(h, walkers[i]));
+    if (walkers[i].n[0] + walkers[i].n[1] == 0)
+      continue;  // empty configuration: no-op (tp_accumulator_cpu.hpp:238-239)
+    todo.push_back(&walkers[i]);
+  }
+  for (size_t i0 = 0; i0 < todo.size(); i0 += h->max_walkers) {
+    const int nw_ = static_cast<int>(std::min<size_t>(h->max_walkers, todo.size() - i0));
+    G4_TRY(stage_walkers(h, todo.data() + i0, nw_, memory));
+    G4_TRY(run_batch(h, nw_));
+  }
+  return G4ACC_OK;
+}
+
+int g4acc_accumulate(g4acc_handle* h, const g4acc_walker* walker, int memory) {
+  return g4acc_accumulate_batch(h, walker, 1, memory);
+}
+
+int g4acc_sync(g4acc_handle* h) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  G4_TRY(h->check(hipStreamSynchronize(h->stream), "hipStreamSynchronize"));
+  h->staged_pending = false;
+  int flag = 0;
+  G4_TRY(h->check(hipMemcpy(&flag, h->d_err, sizeof(int), hipMemcpyDeviceToHost), "read error flag"));
+  if (flag) {
+    G4_TRY(h->check(hipMemset(h->d_err, 0, sizeof(int)), "clear error flag"));
+    return h->fail(G4ACC_INVALID_ARG, "a device vertex had a site or band index out of range");
+  }
+  return G4ACC_OK;
+}
+
+int g4acc_finalize(g4acc_handle* h) { return g4acc_sync(h); }
+
+int g4acc_get_g4(g4acc_handle* h, int channel_index, double* out, size_t n_complex) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  if (!h->g4_present)
+    return h->fail(G4ACC_EMPTY, "There is no G4 stored in this class.");
+  if (channel_index < 0 || channel_index >= static_cast<int>(h->channels.size()) || !out ||
+      n_complex < h->g4_entries)
+    return h->fail(G4ACC_INVALID_ARG, "bad channel index or output buffer too small");
+  G4_TRY(g4acc_sync(h));
+  void* src = h->d_g4[channel_index];
+  if (h->sum_mode == G4ACC_SUM_FP64)
+    return h->check(hipMemcpy(out, src, sizeof(double2) * h->g4_entries, hipMemcpyDeviceToHost), "copy G4");
+  double2* tmp = nullptr;
+  G4_TRY(h->check(hipMalloc(&tmp, sizeof(double2) * h->g4_entries), "hipMalloc(export)"));
+  launch_g4_export(src, tmp, h->g4_entries, h->sum_mode, h->stream);
+  int rc = h->check(hipStreamSynchronize(h->stream), "export G4");
+  if (rc == G4ACC_OK)
+    rc = h->check(hipMemcpy(out, tmp, sizeof(double2) * h->g4_entries, hipMemcpyDeviceToHost), "copy G4");
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int g4acc_sum_to(g4acc_handle* src, g4acc_handle* dst) {
+  if (!src || !dst || src == dst)
+    return G4ACC_INVALID_ARG;
+  if (src->channels != dst->channels || src->g4_entries != dst->g4_entries ||
+      src->sum_mode != dst->sum_mode || src->device != dst->device)
+    return src->fail(G4ACC_INVALID_ARG, "Objects accumulate different number of channels.");
+  if (!src->g4_present || !dst->g4_present)
+    return src->fail(G4ACC_EMPTY, "There is no G4 stored in this class.");
+  G4_TRY(src->check(hipSetDevice(src->device), "hipSetDevice"));
+  G4_TRY(src->check(hipStreamSynchronize(src->stream), "hipStreamSynchronize"));
+  for (size_t c = 0; c < src->channels.size(); ++c)
+    launch_g4_add(dst->d_g4[c], src->d_g4[c], src->g4_entries, src->sum_mode, dst->stream);
+  G4_TRY(dst->check(hipStreamSynchronize(dst->stream), "sum_to"));
+  src->g4_present = false;
+  return G4ACC_OK;
+}
+
+size_t g4acc_g4_size(const g4acc_handle* h) { return h ? h->g4_entries : 0; }
+size_t g4acc_bytes(const g4acc_handle* h) { return h ? h->bytes : 0; }
+void* g4acc_stream(g4acc_handle* h) { return h ? static_cast<void*>(h->stream) : nullptr; }
+
+const char* g4acc_last_error(const g4acc_handle* h) {
+  return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+size_t g4acc_g_size(const g4acc_handle* h) {
+  return h ? 2 * static_cast<size_t>(h->g.D) * h->g.D : 0;
+}
+
+int g4acc_compute_G(g4acc_handle* h, const g4acc_walker* walker, int memory, double* out,
+                    size_t n_complex) {
+  if (!h || !walker || !out)
+    return G4ACC_INVALID_ARG;
+  if (n_complex < g4acc_g_size(h))
+    return h->fail(G4ACC_INVALID_ARG, "output buffer too small");
+  G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  G4_TRY(check_walker(h, *walker));
+  const g4acc_walker* ws[1] = {walker};
+  G4_TRY(stage_walkers(h, ws, 1, memory));
+  if (h->precision == G4ACC_FP64)
+    run_mtog<double>(h, 2);
+  else
+    run_mtog<float>(h, 2);
+  double2* tmp = nullptr;
+  const size_t n = g4acc_g_size(h);
+  G4_TRY(h->check(hipMalloc(&tmp, sizeof(double2) * n), "hipMalloc(export)"));
+  if (h->precision == G4ACC_FP64)
+    launch_export_G<double>(static_cast<const double2*>(h->d_G), tmp, h->g, h->stream);
+  else
+    launch_export_G<float>(static_cast<const float2*>(h->d_G), tmp, h->g, h->stream);
+  int rc = g4acc_sync(h);
+  if (rc == G4ACC_OK)
+    rc = h->check(hipMemcpy(out, tmp, sizeof(double2) * n, hipMemcpyDeviceToHost), "copy G");
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int g4acc_accumulate_G(g4acc_handle* h, const double* G_ref, double sign) {
+  if (!h || !G_ref)
+    return G4ACC_INVALID_ARG;
+  if (!h->initialized)
+    return h->fail(G4ACC_NOT_INITIALIZED, "accumulate called before resetAccumulation");
+  if (!(sign == 1.0 || sign == -1.0))
+    return h->fail(G4ACC_INVALID_ARG, "sign must be +1 or -1");
+  if (!h->g4_present)
+    return G4ACC_OK;
+  G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  const size_t n = g4acc_g_size(h);
+  double2* tmp = nullptr;
+  G4_TRY(h->check(hipMalloc(&tmp, sizeof(double2) * n), "hipMalloc(import)"));
+  int rc = h->check(hipMemcpyAsync(tmp, G_ref, sizeof(double2) * n, hipMemcpyHostToDevice, h->stream),
+                    "copy G");
+  if (rc == G4ACC_OK && h->staged_pending) {
+    rc = h->check(hipEventSynchronize(h->staged), "hipEventSynchronize");
+    h->staged_pending = false;
+  }
+  if (rc == G4ACC_OK) {
+    h->h_desc[0] = SlotDesc{nullptr, nullptr, nullptr, nullptr, 1, 1, sign};
+    h->h_desc[1] = h->h_desc[0];
+    rc = h->check(hipMemcpyAsync(h->d_desc, h->h_desc, 2 * sizeof(SlotDesc), hipMemcpyHostToDevice,
+                                 h->stream), "copy desc");
+  }
+  if (rc == G4ACC_OK) {
+    if (h->precision == G4ACC_FP64) {
+      launch_import_G<double>(tmp, static_cast<double2*>(h->d_G), h->g, h->stream);
+      run_g4<double>(h, 1);
+    }
+    else {
+      launch_import_G<float>(tmp, static_cast<float2*>(h->d_G), h->g, h->stream);
+      run_g4<float>(h, 1);
+    }
+    rc = h->check(hipStreamSynchronize(h->stream), "accumulate_G");
+  }
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int g4acc_set_timing(g4acc_handle* h, int enable) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  h->timing = enable != 0;
+  h->t_mtog_ms = h->t_g4_ms = 0;
+  h->timed_batches = 0;
+  return G4ACC_OK;
+}
+
+int g4acc_get_timing(const g4acc_handle* h, double* mtog_ms, double* g4_ms, long long* batches) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  if (mtog_ms)
+    *mtog_ms = h->t_mtog_ms;
+  if (g4_ms)
+    *g4_ms = h->t_g4_ms;
+  if (batches)
+    *batches = h->timed_batches;
+  return G4ACC_OK;
+}
+
+}  // extern "C"

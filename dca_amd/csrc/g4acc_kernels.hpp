@@ -1,0 +1,65 @@
+// Host-side declarations of the g4acc kernel launchers.
+#pragma once
+
+#include "g4acc_common.hpp"
+
+namespace g4acc {
+
+template <typename Real>
+struct MtoGArgs {
+  Geo g;
+  const SlotDesc* desc;   // n_slots device descriptors
+  int n_slots;            // 2 * walkers
+  int n_max;              // vertex capacity per slot
+  int chunk;              // slots whose S1..S4 workspaces exist at once
+  const int* perm;        // [n_slots][n_max]
+  const int* off;         // [n_slots][n_orb + 1]
+  Cplx<Real>* T;          // [n_slots][n_max][nwE]
+  Cplx<Real>* B;          // [chunk][n_max][D]
+  Cplx<Real>* A;          // [chunk][n_max][D]
+  Cplx<Real>* E;          // [chunk][D][D]
+  Cplx<Real>* G;          // [n_slots][D][D]
+  const Cplx<Real>* dft;  // [Nc][Nc] (k fastest)
+  const double2* bphase;  // [Nc][nb] or nullptr
+  const double2* G0;      // (b1, b2, s, k, wE)
+};
+
+void launch_sort(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, int* perm, int* off,
+                 int* err_flag, hipStream_t st);
+
+template <typename Real>
+void launch_mtog(const MtoGArgs<Real>& a, hipStream_t st);
+
+enum SumMode { SUM_FP64 = 0, SUM_FP32_KAHAN = 1 };
+
+template <typename Real>
+struct G4Args {
+  Geo g;
+  int channel;                 // FourPointType value
+  const Cplx<Real>* G;         // [n_slots][D][D]
+  const SlotDesc* desc;        // signs
+  int n_walkers;
+  const int* k_add;            // device [Nc][Nc]
+  const int* k_sub;
+  const int* k_ex;             // device [nq]
+  const int* w_ex;             // device [nv]
+  int sum_mode;
+  void* g4;                    // double2[size] (FP64) or float4[size] (value, compensation)
+};
+
+template <typename Real>
+void launch_g4(const G4Args<Real>& a, hipStream_t st);
+
+// dst += src over n complex entries of a G4 buffer in the given sum mode.
+void launch_g4_add(void* dst, const void* src, size_t n, int sum_mode, hipStream_t st);
+// G4 buffer (any sum mode) -> complex double.
+void launch_g4_export(const void* src, double2* dst, size_t n, int sum_mode, hipStream_t st);
+// Fill n 16-byte entries with zeros (a kernel, not hipMemsetAsync, so it is safe to capture).
+void launch_zero16(void* dst, size_t n, hipStream_t st);
+// Reference-layout G (b1,b2,s,k1,k2,w1E,w2E) complex double -> per-slot device layout of one walker.
+template <typename Real>
+void launch_import_G(const double2* ref, Cplx<Real>* G, const Geo& g, hipStream_t st);
+template <typename Real>
+void launch_export_G(const Cplx<Real>* G, double2* ref, const Geo& g, hipStream_t st);
+
+}  // namespace g4acc

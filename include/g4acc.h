@@ -1,0 +1,156 @@
+/* g4acc: MI355X (gfx950) two-particle Green's function accumulator, C ABI.
+ *
+ * Drop-in boundary for DCA++'s TpAccumulator<Parameters, DistType, DeviceType>
+ * (include/dca/phys/dca_step/cluster_solver/shared_tools/accumulation/tp/tp_accumulator_base.hpp:50-51,
+ *  CPU: .../tp/tp_accumulator_cpu.hpp:56, GPU: .../tp/tp_accumulator_gpu.hpp:59).
+ * Plain C types only; no function throws; every call returns a g4acc_status.
+ *
+ * Layouts (first index fastest, as dca::func::function stores them):
+ *   G0  (b1, b2, s, k, wE) complex double, wE over the EXTENDED vertex frequencies
+ *   G4  (b1, b2, b3, b4, k1, w1, k2, w2, k_ex, w_ex) complex double per channel
+ *   G   (b1, b2, s, k1, k2, w1E, w2E) complex double (TpAccumulatorBase::G_)
+ *   M   column-major N x N real double per spin sector, leading dimension ld
+ * Complex values are interleaved (re, im).
+ */
+#ifndef G4ACC_H
+#define G4ACC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum g4acc_status {
+  G4ACC_OK = 0,
+  G4ACC_INVALID_ARG = 1,          /* std::logic_error / std::invalid_argument in the reference */
+  G4ACC_UNSUPPORTED_CHANNEL = 2,  /* tp_accumulator_cpu.hpp:663-664, tp_accumulator_gpu.hpp:456-468 */
+  G4ACC_NOT_INITIALIZED = 3,      /* accumulate before resetAccumulation, tp_accumulator_gpu.hpp:329-330 */
+  G4ACC_OUT_OF_MEMORY = 4,        /* std::bad_alloc, tp_accumulator_gpu.hpp:314-317 */
+  G4ACC_DEVICE_ERROR = 5,         /* std::runtime_error on a failed launch, tp_accumulator_kernels.cu:695-700 */
+  G4ACC_EMPTY = 6                 /* get_G4 with no G4 stored, tp_accumulator_cpu.hpp:874-875 */
+} g4acc_status;
+
+/* FourPointType values (include/dca/phys/four_point_type.hpp:27-33). */
+typedef enum g4acc_channel {
+  G4ACC_PARTICLE_HOLE_TRANSVERSE = 1,
+  G4ACC_PARTICLE_HOLE_MAGNETIC = 2,
+  G4ACC_PARTICLE_HOLE_CHARGE = 3,
+  G4ACC_PARTICLE_PARTICLE_UP_DOWN = 4
+} g4acc_channel;
+
+/* Arithmetic of the M -> G step. The reference's choice is TpAccumulationPrecision
+ * (include/dca/config/mc_options.hpp.in:28, float or double). */
+typedef enum g4acc_precision {
+  G4ACC_FP64 = 0,   /* complex128 G, FP64 products (MFMA f64) */
+  G4ACC_FP32 = 1    /* complex64 G, FP32 products with FP32 accumulation (MFMA f32) */
+} g4acc_precision;
+
+/* Running sum of G4 over measurements. */
+typedef enum g4acc_sum_mode {
+  G4ACC_SUM_FP64 = 0,        /* FP64 G4 */
+  G4ACC_SUM_FP32_KAHAN = 1   /* FP32 G4 with a per-entry FP32 Kahan compensation term */
+} g4acc_sum_mode;
+
+typedef enum g4acc_memory {
+  G4ACC_HOST = 0,    /* walker arrays are host memory (copied to the device by the call) */
+  G4ACC_DEVICE = 1   /* walker arrays are device memory of the accumulator's device */
+} g4acc_memory;
+
+/* Everything the reference constructor reads from Parameters, the static domains and G0
+ * (TpAccumulatorBase ctor, tp_accumulator_base.hpp:184-221). */
+typedef struct g4acc_geometry {
+  int n_bands;             /* BANDS of the model */
+  int n_sites;             /* cluster sites Nc (= momenta) */
+  double beta;
+  int n_freq;              /* COMPACT vertex frequencies = 2 * four-point-fermionic-frequencies */
+  int n_w_ex;              /* FrequencyExchangeDomain elements */
+  const int* w_ex;
+  int n_k_ex;              /* MomentumExchangeDomain elements (k indices) */
+  const int* k_ex;
+  int n_channels;          /* channels, sorted and unique (four_point_parameters.hpp:194-196) */
+  const int* channels;
+  const int* k_add;        /* Nc*Nc, k_add[i + Nc*j] = index(k_i + k_j) */
+  const int* k_sub;        /* Nc*Nc, k_sub[i + Nc*j] = index(k_j - k_i) */
+  const double* dft;       /* Nc*Nc complex, dft[k + Nc*r] = exp(i k.r) */
+  const double* band_phase;/* n_bands*Nc complex exp(i k.a_b) at [b + n_bands*k], or NULL */
+  const double* g0;        /* n_bands^2 * 2 * Nc * (n_freq + 2 max|w_ex|) complex, (b1,b2,s,k,wE) */
+} g4acc_geometry;
+
+typedef struct g4acc_options {
+  int precision;      /* g4acc_precision */
+  int sum_mode;       /* g4acc_sum_mode */
+  int max_walkers;    /* walkers per batched launch (>= 1) */
+  int max_vertices;   /* largest N per spin sector */
+  int device;         /* HIP device ordinal */
+} g4acc_options;
+
+/* One measurement: what a walker hands to TpAccumulator::accumulate(M_pair, configs, sign)
+ * (tp_accumulator_cpu.hpp:107-109). Density-density vertices: one band and one site each
+ * (Vertex::get_left_band()/get_right_band() agree). */
+typedef struct g4acc_walker {
+  int n[2];                 /* vertices per spin sector (configs[s].size()) */
+  const double* m[2];       /* M_pair[s], column-major */
+  int ld[2];                /* leading dimensions of m[s] (>= n[s]) */
+  const double* tau[2];     /* vertex imaginary times in [0, beta) */
+  const int* site[2];       /* vertex cluster sites (r indices) */
+  const int* band[2];       /* vertex bands */
+  double sign;              /* SignType: +1 / -1 */
+} g4acc_walker;
+
+typedef struct g4acc_handle g4acc_handle;
+
+/* Constructor (tp_accumulator_base.hpp:184-208, tp_accumulator_gpu_base.hpp:132-152). */
+int g4acc_create(const g4acc_geometry* geometry, const g4acc_options* options, g4acc_handle** out);
+/* Destructor. */
+int g4acc_destroy(g4acc_handle* h);
+/* resetAccumulation (tp_accumulator_cpu.hpp:224-229, tp_accumulator_gpu.hpp:271-296). */
+int g4acc_reset(g4acc_handle* h);
+/* accumulate(M_pair, configs, sign) (tp_accumulator_cpu.hpp:231-252, tp_accumulator_gpu.hpp:321-355).
+ * An empty configuration (n[0] + n[1] == 0) is a no-op, as on the CPU (:238-239).
+ * Caller buffers are borrowed until g4acc_sync (the reference's synchronizeCopy). */
+int g4acc_accumulate(g4acc_handle* h, const g4acc_walker* walker, int memory);
+/* Many walkers' measurements in one batched launch sequence; equivalent to calling
+ * g4acc_accumulate for each walker. n_walkers may exceed max_walkers (processed in chunks). */
+int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_walkers, int memory);
+/* finalize() (tp_accumulator_gpu.hpp:471-484): waits for all queued work. */
+int g4acc_finalize(g4acc_handle* h);
+/* synchronizeCopy()/syncStreams(): after this returns, caller buffers may be reused. */
+int g4acc_sync(g4acc_handle* h);
+/* get_G4()[channel_index] (tp_accumulator_cpu.hpp:871-877): copies n_complex = g4acc_g4_size()
+ * complex doubles in the reference order. */
+int g4acc_get_g4(g4acc_handle* h, int channel_index, double* out, size_t n_complex);
+/* sumTo(other) (tp_accumulator_cpu.hpp:895-904): dst.G4 += src.G4 on the device, then src.G4
+ * is cleared (get_g4 on src returns G4ACC_EMPTY until the next g4acc_reset). */
+int g4acc_sum_to(g4acc_handle* src, g4acc_handle* dst);
+/* Complex entries per channel of G4. */
+size_t g4acc_g4_size(const g4acc_handle* h);
+/* Device bytes held by the handle. */
+size_t g4acc_bytes(const g4acc_handle* h);
+/* get_stream() (tp_accumulator_gpu.hpp): the hipStream_t the handle launches on. */
+void* g4acc_stream(g4acc_handle* h);
+/* Message of the last failed call on h (or of the last failed g4acc_create when h is NULL). */
+const char* g4acc_last_error(const g4acc_handle* h);
+
+/* Testing entry (the reference's get_G_Debug, tp_accumulator_cpu.hpp:120-123): runs only the
+ * M -> G step of one walker and copies G in the reference layout (b1,b2,s,k1,k2,w1E,w2E). */
+int g4acc_compute_G(g4acc_handle* h, const g4acc_walker* walker, int memory, double* out,
+                    size_t n_complex);
+/* Complex entries of G in the reference layout. */
+size_t g4acc_g_size(const g4acc_handle* h);
+/* Testing entry (the minimum slice of the path): accumulates one measurement whose G is given
+ * in the reference layout (b1,b2,s,k1,k2,w1E,w2E), host memory, skipping M -> G. Same effect on
+ * G4 as the reference's updateG4 loop over channels (tp_accumulator_cpu.hpp:248-249). */
+int g4acc_accumulate_G(g4acc_handle* h, const double* G, double sign);
+
+/* Profiling hook (the reference's Profiler scopes "accumulate"/"updateG4",
+ * tp_accumulator_cpu.hpp:236,416): when enabled, every batch records HIP events around its
+ * M -> G and G4 stages and waits for them; times accumulate until the next enable call. */
+int g4acc_set_timing(g4acc_handle* h, int enable);
+int g4acc_get_timing(const g4acc_handle* h, double* mtog_ms, double* g4_ms, long long* batches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* G4ACC_H */

@@ -1,0 +1,185 @@
+"""Parity of the HIP path (dca_amd/libg4acc.so through the C ABI) against the FP64 CPU oracle.
+
+Tolerances (max-normalised l_inf of include/dca/function/util/difference.hpp:71-106):
+  FP64 path            1e-12 (G and G4)
+  FP32 path (G, G4)    1e-6  (the accuracy gate of the mixed-precision form)
+The reference's own CPU-vs-GPU bar is 5e-7 (TT/tp/tp_accumulator_gpu_test.cpp:213).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from dca_amd import geometry as geo
+from dca_amd import g4acc, synth
+from oracle import bruteforce as bf
+
+pytestmark = pytest.mark.gpu
+
+ALL_CHANNELS = [geo.PARTICLE_HOLE_TRANSVERSE, geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE,
+                geo.PARTICLE_PARTICLE_UP_DOWN]
+
+CASES = {
+    # name: (nb, L or cluster, n_freq, w_ex, k_ex, n_range)
+    "1band_2x2": (1, 2, 4, (0, 1), ((0.0, 0.0), (math.pi, math.pi)), (1, 9)),
+    "1band_3x3_shift2": (1, 3, 6, (0, 1, 2), ((0.0, 0.0), (2 * math.pi / 3, 0.0)), (3, 12)),
+    "1band_2x2_negative_wex": (1, 2, 4, (-2,), ((math.pi, 0.0),), (1, 8)),
+    "3band_2x2": (3, 2, 4, (0, 1), ((0.0, 0.0), (math.pi, math.pi)), (2, 10)),
+    "2band_8site": (2, "8site", 4, (0, 1), ((0.0, 0.0), (math.pi, math.pi)), (2, 12)),
+}
+
+
+def case(name, channels=ALL_CHANNELS):
+    nb, L, nf, wex, kex, nr = CASES[name]
+    cl = geo.cluster_2d((2, 2), (2, -2)) if L == "8site" else geo.square_cluster(L)
+    return synth.tiny_workload(nb=nb, cluster=cl, n_freq=nf, w_ex=wex, k_ex=kex, channels=channels,
+                               n_range=nr, seed_cfg=200 + list(CASES).index(name))
+
+
+def linf(ref, x):
+    return bf.difference(ref, x)["l_inf"]
+
+
+def oracle_g4(oracle_lib, geom, measurements):
+    acc = oracle_lib.OracleAccumulator(geom)
+    for m in measurements:
+        acc.accumulate(m)
+    return acc.get_G4()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-6)])
+def test_G_matches_oracle(oracle_lib, name, precision, tol):
+    wl = case(name)
+    acc = g4acc.TpAccumulator(wl.geometry, precision=precision, max_vertices=wl.n_max)
+    ora = oracle_lib.OracleAccumulator(wl.geometry)
+    for meas in wl.draw(3):
+        ora.compute_G(meas)
+        ref = ora.get_G()
+        got = acc.compute_G(meas)
+        err = linf(ref, got)
+        print(f"{name} {precision} G l_inf={err:.3e}", flush=True)
+        assert err < tol
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_G4_kernel_from_oracle_G(oracle_lib, name):
+    """The G4 update alone: feed the oracle's G, compare every channel (FP64)."""
+    wl = case(name)
+    g = wl.geometry
+    acc = g4acc.TpAccumulator(g, precision="fp64", max_vertices=wl.n_max)
+    acc.resetAccumulation()
+    ora = oracle_lib.OracleAccumulator(g)
+    for meas in wl.draw(3):
+        ora.accumulate(meas)
+        acc.accumulate_G(ora.get_G(), meas.sign)
+    for ch, ref, got in zip(g.channels, ora.get_G4(), acc.get_G4()):
+        err = linf(ref, got)
+        print(f"{name} {geo.CHANNEL_NAMES[ch]} l_inf={err:.3e}", flush=True)
+        assert err < 1e-13
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("precision,sum_mode,tol", [("fp64", "fp64", 1e-12), ("fp32", "fp64", 1e-6),
+                                                     ("fp32", "fp32_kahan", 1e-6)])
+def test_accumulate_matches_oracle(oracle_lib, name, precision, sum_mode, tol):
+    wl = case(name)
+    g = wl.geometry
+    ms = wl.draw(4)
+    ref = oracle_g4(oracle_lib, g, ms)
+    acc = g4acc.TpAccumulator(g, precision=precision, sum_mode=sum_mode, max_vertices=wl.n_max)
+    acc.resetAccumulation()
+    for m in ms:
+        acc.accumulate(m)
+    acc.finalize()
+    for ch, r, x in zip(g.channels, ref, acc.get_G4()):
+        err = linf(r, x)
+        print(f"{name} {precision}/{sum_mode} {geo.CHANNEL_NAMES[ch]} l_inf={err:.3e}", flush=True)
+        assert err < tol
+
+
+@pytest.mark.parametrize("name", ["1band_3x3_shift2", "3band_2x2"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batched_equals_sequential(oracle_lib, name, precision):
+    wl = case(name)
+    g = wl.geometry
+    ms = wl.draw(7)
+    seq = g4acc.TpAccumulator(g, precision=precision, max_vertices=wl.n_max)
+    seq.resetAccumulation()
+    for m in ms:
+        seq.accumulate(m)
+    bat = g4acc.TpAccumulator(g, precision=precision, max_walkers=3, max_vertices=wl.n_max)
+    bat.resetAccumulation()
+    bat.accumulate_batch(ms)  # 7 walkers through a 3-walker launch: chunks of 3, 3, 1
+    for a, b in zip(seq.get_G4(), bat.get_G4()):
+        assert linf(a, b) < (1e-13 if precision == "fp64" else 1e-6)
+    ref = oracle_g4(oracle_lib, g, ms)
+    for r, b in zip(ref, bat.get_G4()):
+        assert linf(r, b) < (1e-12 if precision == "fp64" else 1e-6)
+
+
+def test_sign_linearity_and_empty_configuration():
+    wl = case("1band_2x2")
+    g = wl.geometry
+    m = wl.draw(1)[0]
+    acc = g4acc.TpAccumulator(g, max_vertices=wl.n_max)
+    acc.resetAccumulation()
+    m.sign = 1.0
+    acc.accumulate(m)
+    plus = acc.get_G4()
+    acc.resetAccumulation()
+    m.sign = -1.0
+    acc.accumulate(m)
+    minus = acc.get_G4()
+    for p, q in zip(plus, minus):
+        assert np.max(np.abs(p + q)) <= 1e-14 * np.abs(p).max()
+    empty = synth.Measurement(n=[0, 0], M=[np.zeros((0, 0))] * 2, tau=[np.zeros(0)] * 2,
+                              site=[np.zeros(0, np.int32)] * 2, band=[np.zeros(0, np.int32)] * 2,
+                              sign=1.0)
+    acc.accumulate(empty)
+    for a, b in zip(minus, acc.get_G4()):
+        assert np.array_equal(a, b)
+
+
+def test_one_empty_spin_sector(oracle_lib):
+    wl = case("3band_2x2")
+    g = wl.geometry
+    m = wl.draw(1)[0]
+    m.n[1] = 0
+    m.M[1] = np.zeros((0, 0))
+    m.tau[1], m.site[1], m.band[1] = np.zeros(0), np.zeros(0, np.int32), np.zeros(0, np.int32)
+    ref = oracle_g4(oracle_lib, g, [m])
+    acc = g4acc.TpAccumulator(g, max_vertices=wl.n_max)
+    acc.resetAccumulation()
+    acc.accumulate(m)
+    for r, x in zip(ref, acc.get_G4()):
+        assert linf(r, x) < 1e-12
+
+
+def test_sum_to_and_errors(oracle_lib):
+    wl = case("1band_2x2")
+    g = wl.geometry
+    m1, m2 = wl.draw(2)
+    a = g4acc.TpAccumulator(g, max_vertices=wl.n_max)
+    b = g4acc.TpAccumulator(g, max_vertices=wl.n_max)
+    with pytest.raises(g4acc.LogicError, match="before resetAccumulation"):
+        a.accumulate(m1)
+    a.resetAccumulation()
+    b.resetAccumulation()
+    a.accumulate(m1)
+    b.accumulate(m2)
+    a.sumTo(b)
+    ref = oracle_g4(oracle_lib, g, [m1, m2])
+    for r, x in zip(ref, b.get_G4()):
+        assert linf(r, x) < 1e-12
+    with pytest.raises(g4acc.LogicError, match="no G4"):
+        a.get_G4()
+    big = wl.draw(1)[0]
+    big.n[0] = wl.n_max + 1
+    with pytest.raises(g4acc.LogicError, match="max_vertices"):
+        b.accumulate(big)
+    bad = wl.draw(1)[0]
+    bad.site[0] = bad.site[0].copy()
+    bad.site[0][0] = g.nc  # out of range
+    with pytest.raises(g4acc.LogicError, match="out of range"):
+        b.accumulate(bad)
