@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=8, help="measurements timed on the CPU oracle")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-calibration", type=int, default=0,
+                   help="bytes of a plain device-to-device copy run first (counter calibration)")
     return p.parse_args()
 
 
@@ -133,6 +135,14 @@ def main():
     from dca_amd import g4acc, synth
     from oracle import bruteforce as bf
 
+    if args.copy_calibration:
+        n = args.copy_calibration // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {n * 8} bytes done")
+        del src, dst
     wl = synth.workload(args.config)
     g = wl.geometry
     W = args.walkers or wl.walkers

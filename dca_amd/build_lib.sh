@@ -1,11 +1,21 @@
 #!/bin/bash
 # Builds dca_amd/libg4acc.so for gfx950 with hipcc (cross-compiles without a GPU).
+# Each source compiles to its own object in parallel; objects go to dca_amd/_build/.
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 OUT="$HERE/libg4acc.so"
-SRC=("$HERE"/csrc/g4acc_api.hip "$HERE"/csrc/mtog_kernels.hip "$HERE"/csrc/g4_kernels.hip)
-FLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared -Wall
+OBJ="$HERE/_build"
+mkdir -p "$OBJ"
+SRC=(g4acc_api mtog_kernels mtog_fused g4_kernels g4_tiled)
+FLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Wall
        -Xclang -target-feature -Xclang -packed-fp32-ops)
-/opt/rocm/bin/hipcc "${FLAGS[@]}" "${SRC[@]}" -o "$OUT.tmp" 2> >(grep -v "packed-fp32-ops' is not a recognized feature" >&2)
+filter() { grep -v "packed-fp32-ops' is not a recognized feature" >&2 || true; }
+pids=()
+for s in "${SRC[@]}"; do
+  /opt/rocm/bin/hipcc "${FLAGS[@]}" -c "$HERE/csrc/$s.hip" -o "$OBJ/$s.o" 2> >(filter) &
+  pids+=($!)
+done
+for p in "${pids[@]}"; do wait "$p"; done
+/opt/rocm/bin/hipcc --offload-arch=gfx950 -shared -fPIC $(printf "%s.o " "${SRC[@]/#/$OBJ/}") -o "$OUT.tmp" 2> >(filter)
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"

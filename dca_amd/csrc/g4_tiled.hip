@@ -1,0 +1,379 @@
+// Single-band G4 update, LDS-tiled: all channels of a batch of walkers in one launch.
+//
+// Same sums as g4_kernels.hip (tp_accumulator_cpu.hpp:407-666) for n_bands == 1, organised around
+// what the entries of one tile share. A workgroup owns the tile
+//     { (k1, w1, k2, w2, q, v) : all k1, k2;  w1, w2 fixed;  v fixed;  q in a group of QG }
+// and, per walker and spin, reads
+//   X(k1,k2)    = G[(k1, w1), (k2, w2)]                        directly (column segments)
+//   Yb(a,b)     = G[(a, w2+v), (b, w1+v)]                      into LDS: Y^q(k1,k2) = Yb(k2+q, k1+q)
+//   Zb(a,b)     = G[(a, v-w1), (b, v-w2)]                      into LDS: Z^q(k1,k2) = Zb(q-k1, q-k2)
+//   du,su(k1)   = a (u_0 -+ u_1),  u_s = G_s[(k1,w1), (k1+q, w1+v)]   precomputed (uv_kernel)
+//   dv,sv(k2)   =    v_0 -+ v_1,   v_s = G_s[(k2+q,w2+v), (k2,w2)]
+// so every G element a tile needs is fetched once per walker and reused by the QG momenta.
+//   MAGNETIC    += du dv - a (X_0 Y_0 + X_1 Y_1)
+//   CHARGE      += su sv - a (X_0 Y_0 + X_1 Y_1)
+//   PP_UP_DOWN  += a (X_0 Z_1 + X_1 Z_0)
+//   TRANSVERSE  -= a (X_0 Y_1 + X_1 Y_0)
+// Products are formed in the G precision; each walker's term is added to FP64 accumulators;
+// G4 is read and written once per launch (FP64, or FP32 value + Kahan compensation).
+#include "g4acc_common.hpp"
+#include "g4acc_kernels.hpp"
+
+namespace g4acc {
+
+namespace {
+constexpr int PH_MAGNETIC = 2, PH_CHARGE = 3, PP_UP_DOWN = 4;  // else: PH_TRANSVERSE (1)
+constexpr int TILE_THREADS = 256;
+
+__device__ __forceinline__ void g4_store_t(void* g4, size_t i, double2 acc, int sum_mode) {
+  if (sum_mode == SUM_FP64) {
+    double2* p = static_cast<double2*>(g4) + i;
+    double2 v = *p;
+    v.x += acc.x;
+    v.y += acc.y;
+    *p = v;
+  }
+  else {
+    float4* p = static_cast<float4*>(g4) + i;
+    float4 v = *p;
+    const float hx = static_cast<float>(acc.x), hy = static_cast<float>(acc.y);
+    const float lx = static_cast<float>(acc.x - hx), ly = static_cast<float>(acc.y - hy);
+    const float yx = hx - v.z, yy = hy - v.w;
+    const float tx = v.x + yx, ty = v.y + yy;
+    v.z = ((tx - v.x) - yx) - lx;
+    v.w = ((ty - v.y) - yy) - ly;
+    v.x = tx;
+    v.y = ty;
+    *p = v;
+  }
+}
+}  // namespace
+
+// UV[((m * 4 + plane) * nqv + qv) * P + p], planes: a(u0-u1), a(u0+u1), v0-v1, v0+v1.
+template <typename Real>
+__global__ void __launch_bounds__(256) uv_kernel(TiledArgs<Real> a) {
+  using C = Cplx<Real>;
+  const Geo& g = a.g;
+  const int P = g.nc * g.nw;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P)
+    return;
+  const int qv = blockIdx.y, m = blockIdx.z;
+  const int nqv = g.nq * g.nv;
+  const int q = a.k_ex[qv % g.nq], nu = a.w_ex[qv / g.nq];
+  const int k = p % g.nc, w = p / g.nc;
+  const int kq = a.k_add[k + g.nc * q];
+  const size_t D = g.D;
+  const C* G0 = a.G + (2 * static_cast<size_t>(m)) * D * D;
+  const C* G1 = G0 + D * D;
+  const size_t ru = k + static_cast<size_t>(g.nc) * (w + g.ext);
+  const size_t cu = kq + static_cast<size_t>(g.nc) * (w + nu + g.ext);
+  const C u0 = G0[ru + D * cu], u1 = G1[ru + D * cu];
+  const C v0 = G0[cu + D * ru], v1 = G1[cu + D * ru];
+  const Real alpha = static_cast<Real>(0.5 * a.desc[2 * m].sign);
+  C* out = a.UV + (static_cast<size_t>(m) * 4 * nqv + qv) * P + p;
+  const size_t plane = static_cast<size_t>(nqv) * P;
+  out[0] = C{alpha * (u0.x - u1.x), alpha * (u0.y - u1.y)};
+  out[plane] = C{alpha * (u0.x + u1.x), alpha * (u0.y + u1.y)};
+  out[2 * plane] = C{v0.x - v1.x, v0.y - v1.y};
+  out[3 * plane] = C{v0.x + v1.x, v0.y + v1.y};
+}
+
+template <typename Real, int PPT, int QG, int NCH>
+__global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> a) {
+  using C = Cplx<Real>;
+  extern __shared__ unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  const Geo& g = a.g;
+  const int nc = g.nc, nw = g.nw, e = g.ext, P = nc * nw;
+  const size_t D = g.D, DD = D * D;
+  const int nz = g.nv * a.nqg;
+  int w1, w2, z;
+  if (a.n_xcd > 0) {
+    // Workgroups are dealt to the XCDs round-robin by id; give each XCD a contiguous range of
+    // logical tiles, ordered by super-tiles of ST x ST (w1, w2) blocks with the exchange
+    // frequency and momentum group innermost.
+    const int ST = a.super_tile, nsw = a.n_super;
+    const unsigned n_tiles = gridDim.x;
+    const unsigned xcd = blockIdx.x % a.n_xcd, local = blockIdx.x / a.n_xcd;
+    const unsigned L = xcd * (n_tiles / a.n_xcd) + local;
+    const unsigned per_super = static_cast<unsigned>(ST) * ST * nz;
+    const unsigned sup = L / per_super, rem = L % per_super;
+    z = rem % nz;
+    w1 = (sup % nsw) * ST + (rem / nz) % ST;
+    w2 = (sup / nsw) * ST + (rem / nz) / ST;
+  }
+  else {
+    // Natural order: w1 fastest, then w2, then (v, momentum group).
+    w1 = blockIdx.x % nw;
+    w2 = (blockIdx.x / nw) % nw;
+    z = blockIdx.x / (nw * nw);
+  }
+  if (w1 >= nw || w2 >= nw)
+    return;
+  const int iv = z / a.nqg, q0 = (z % a.nqg) * QG;
+  const int nu = a.w_ex[iv];
+  const int ld = nc + 1, bs = nc * ld;
+  const bool needY = a.need_y, needZ = a.need_z;
+  C* Yl = lds;                        // [spin][b][a], a fastest
+  C* Zl = Yl + (needY ? 2 * bs : 0);
+  C* Ul = Zl + (needZ ? 2 * bs : 0);  // [plane][qi][k]: a(u0-u1), a(u0+u1), v0-v1, v0+v1
+  const size_t xoff = nc * static_cast<size_t>(w1 + e) + D * (nc * static_cast<size_t>(w2 + e));
+  const size_t yoff = nc * static_cast<size_t>(w2 + nu + e) + D * (nc * static_cast<size_t>(w1 + nu + e));
+  const size_t zoff = nc * static_cast<size_t>(nu + nw - 1 - w1 + e) +
+                      D * (nc * static_cast<size_t>(nu + nw - 1 - w2 + e));
+  const int nqv = g.nq * g.nv;
+
+  int k1[PPT], k2[PPT];
+  int yi[PPT][QG], zi[PPT][QG];
+  bool ok[PPT];
+  int qv[QG];
+  bool qok[QG];
+#pragma unroll
+  for (int qi = 0; qi < QG; ++qi) {
+    qok[qi] = q0 + qi < g.nq;
+    qv[qi] = (qok[qi] ? q0 + qi : 0) + g.nq * iv;
+  }
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int pi = threadIdx.x + TILE_THREADS * j;
+    ok[j] = pi < nc * nc;
+    k1[j] = ok[j] ? pi % nc : 0;
+    k2[j] = ok[j] ? pi / nc : 0;
+#pragma unroll
+    for (int qi = 0; qi < QG; ++qi) {
+      const int q = a.k_ex[qok[qi] ? q0 + qi : 0];
+      yi[j][qi] = a.k_add[k1[j] + nc * q] * ld + a.k_add[k2[j] + nc * q];
+      zi[j][qi] = a.k_sub[k2[j] + nc * q] * ld + a.k_sub[k1[j] + nc * q];
+    }
+  }
+  // The rank-one factors: one staged element per thread (4 QG nc <= 256).
+  const int n_uv = 4 * QG * nc;
+  const int u_plane = threadIdx.x / (QG * nc), u_qi = (threadIdx.x / nc) % QG, u_k = threadIdx.x % nc;
+  const bool u_ok = threadIdx.x < n_uv;
+  const size_t plane = static_cast<size_t>(nqv) * P;
+  const size_t u_src = u_plane * plane + static_cast<size_t>(qv[0] + u_qi) * P + u_k +
+                       static_cast<size_t>(nc) * (u_plane < 2 ? w1 : w2);
+  const bool u_valid = u_ok && (q0 + u_qi < g.nq);
+
+  double2 acc[PPT][QG][NCH];
+#pragma unroll
+  for (int j = 0; j < PPT; ++j)
+#pragma unroll
+    for (int qi = 0; qi < QG; ++qi)
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+        acc[j][qi][c] = double2{0, 0};
+
+  // Everything walker m needs is loaded into registers while walker m - 1 is consumed: the
+  // staged blocks (each thread stages at most PPT elements per block, nc^2 <= 256 PPT), the
+  // rank-one factors and the thread's own X entries.
+  C pre[4][PPT], pre_uv, pre_x[2][PPT];
+  auto prefetch = [&](int m) {
+    const C* H0 = a.G + (2 * static_cast<size_t>(m)) * DD;
+    const C* H1 = H0 + DD;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int idx = threadIdx.x + TILE_THREADS * j;
+      if (idx < nc * nc) {
+        const size_t go = (idx % nc) + D * (idx / nc);
+        if (needY) {
+          pre[0][j] = H0[yoff + go];
+          pre[1][j] = H1[yoff + go];
+        }
+        if (needZ) {
+          pre[2][j] = H0[zoff + go];
+          pre[3][j] = H1[zoff + go];
+        }
+      }
+      if (ok[j]) {
+        const size_t xo = xoff + k1[j] + D * k2[j];
+        pre_x[0][j] = H0[xo];
+        pre_x[1][j] = H1[xo];
+      }
+    }
+    if (u_valid)
+      pre_uv = a.UV[static_cast<size_t>(m) * 4 * plane + u_src];
+  };
+  if (a.n_walkers > 0)
+    prefetch(0);
+  for (int m = 0; m < a.n_walkers; ++m) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int idx = threadIdx.x + TILE_THREADS * j;
+      if (idx < nc * nc) {
+        const int li = (idx / nc) * ld + (idx % nc);
+        if (needY) {
+          Yl[li] = pre[0][j];
+          Yl[bs + li] = pre[1][j];
+        }
+        if (needZ) {
+          Zl[li] = pre[2][j];
+          Zl[bs + li] = pre[3][j];
+        }
+      }
+    }
+    if (u_valid)
+      Ul[threadIdx.x] = pre_uv;
+    C xs[2][PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      xs[0][j] = pre_x[0][j];
+      xs[1][j] = pre_x[1][j];
+    }
+    __syncthreads();
+    const Real alpha = static_cast<Real>(0.5 * a.desc[2 * m].sign);
+    if (m + 1 < a.n_walkers)
+      prefetch(m + 1);
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      if (!ok[j])
+        continue;
+      C x0 = xs[0][j], x1 = xs[1][j];
+      x0.x *= alpha;
+      x0.y *= alpha;
+      x1.x *= alpha;
+      x1.y *= alpha;
+#pragma unroll
+      for (int qi = 0; qi < QG; ++qi) {
+        if (!qok[qi])
+          continue;
+        C cross{0, 0}, ppz{0, 0}, trans{0, 0};
+        if (needY) {
+          const C y0 = Yl[yi[j][qi]], y1 = Yl[bs + yi[j][qi]];
+          cross = cadd(cmul(x0, y0), cmul(x1, y1));
+          if (a.has_trans)
+            trans = cadd(cmul(x0, y1), cmul(x1, y0));
+        }
+        if (needZ) {
+          const C z0 = Zl[zi[j][qi]], z1 = Zl[bs + zi[j][qi]];
+          ppz = cadd(cmul(x0, z1), cmul(x1, z0));
+        }
+        const C* uq = Ul + qi * nc;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          C t;
+          const int ch = a.ch[c];
+          if (ch == PH_MAGNETIC)
+            t = csub(cmul(uq[k1[j]], uq[2 * QG * nc + k2[j]]), cross);
+          else if (ch == PH_CHARGE)
+            t = csub(cmul(uq[QG * nc + k1[j]], uq[3 * QG * nc + k2[j]]), cross);
+          else if (ch == PP_UP_DOWN)
+            t = ppz;
+          else
+            t = C{-trans.x, -trans.y};
+          acc[j][qi][c].x += static_cast<double>(t.x);
+          acc[j][qi][c].y += static_cast<double>(t.y);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    if (!ok[j])
+      continue;
+#pragma unroll
+    for (int qi = 0; qi < QG; ++qi) {
+      if (!qok[qi])
+        continue;
+      const size_t idx = k1[j] + static_cast<size_t>(nc) *
+                                     (w1 + static_cast<size_t>(nw) *
+                                               (k2[j] + static_cast<size_t>(nc) *
+                                                            (w2 + static_cast<size_t>(nw) * qv[qi])));
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+        g4_store_t(a.g4[c], idx, acc[j][qi][c], a.sum_mode);
+    }
+  }
+}
+
+namespace {
+
+template <typename Real, int PPT, int QG, int NCH>
+void launch_tiled_fixed(TiledArgs<Real> a, size_t lds, hipStream_t st) {
+  const Geo& g = a.g;
+  unsigned tiles;
+  if (a.tile_order == 1) {
+    a.super_tile = g.nw < 8 ? g.nw : 8;
+    a.n_super = (g.nw + a.super_tile - 1) / a.super_tile;
+    tiles = static_cast<unsigned>(a.n_super * a.super_tile) * (a.n_super * a.super_tile) * g.nv * a.nqg;
+    a.n_xcd = tiles % 8 == 0 ? 8 : 1;
+  }
+  else {
+    a.super_tile = a.n_super = 1;
+    a.n_xcd = 0;
+    tiles = static_cast<unsigned>(g.nw) * g.nw * g.nv * a.nqg;
+  }
+  hipLaunchKernelGGL((g4_tiled_kernel<Real, PPT, QG, NCH>), dim3(tiles), dim3(TILE_THREADS), lds, st, a);
+}
+
+template <typename Real, int PPT, int QG>
+bool launch_tiled_nch(const TiledArgs<Real>& a, size_t lds, hipStream_t st) {
+  switch (a.n_ch) {
+    case 1: launch_tiled_fixed<Real, PPT, QG, 1>(a, lds, st); return true;
+    case 2: launch_tiled_fixed<Real, PPT, QG, 2>(a, lds, st); return true;
+    case 3: launch_tiled_fixed<Real, PPT, QG, 3>(a, lds, st); return true;
+    case 4: launch_tiled_fixed<Real, PPT, QG, 4>(a, lds, st); return true;
+    default: return false;
+  }
+}
+
+}  // namespace
+
+int tiled_pairs_per_thread(int nc) {
+  const int pairs = nc * nc;
+  for (int p : {1, 2, 4, 6, 8, 16})
+    if (p * TILE_THREADS >= pairs)
+      return p;
+  return 0;
+}
+
+size_t tiled_lds_bytes(const Geo& g, bool need_y, bool need_z, size_t cplx_bytes) {
+  const size_t bs = static_cast<size_t>(g.nc) * (g.nc + 1);
+  return cplx_bytes * bs * 2 * ((need_y ? 1 : 0) + (need_z ? 1 : 0));
+}
+
+template <typename Real>
+bool launch_g4_tiled(TiledArgs<Real> a, hipStream_t st) {
+  const Geo& g = a.g;
+  const int ppt = tiled_pairs_per_thread(g.nc);
+  if (g.nb != 1 || ppt == 0 || a.n_ch < 1 || a.n_ch > 4)
+    return false;
+  // Momenta per workgroup: as many as fit 16 FP64 accumulators per thread.
+  int qg = 1;
+  if (ppt == 1)
+    qg = g.nq >= 4 ? 4 : (g.nq >= 2 ? 2 : 1);
+  else if (ppt == 2)
+    qg = g.nq >= 2 ? 2 : 1;
+  while (qg > 1 && ppt * qg * a.n_ch > 16)
+    qg /= 2;
+  if (ppt * qg * a.n_ch > 24 || 4 * qg * g.nc > TILE_THREADS)
+    return false;  // accumulators would not fit in registers / one staged factor per thread
+  a.nqg = (g.nq + qg - 1) / qg;
+  const size_t lds = tiled_lds_bytes(g, a.need_y, a.need_z, sizeof(Cplx<Real>)) +
+                     sizeof(Cplx<Real>) * 4 * qg * g.nc;
+  const dim3 ug((g.nc * g.nw + 255) / 256, g.nq * g.nv, a.n_walkers);
+  hipLaunchKernelGGL(uv_kernel<Real>, ug, dim3(256), 0, st, a);
+#define G4_TILED_CASE(PPT_)                                                   \
+  case PPT_:                                                                  \
+    if (qg == 4) return launch_tiled_nch<Real, PPT_, (PPT_ <= 1 ? 4 : 1)>(a, lds, st); \
+    if (qg == 2) return launch_tiled_nch<Real, PPT_, (PPT_ <= 2 ? 2 : 1)>(a, lds, st); \
+    return launch_tiled_nch<Real, PPT_, 1>(a, lds, st);
+  switch (ppt) {
+    G4_TILED_CASE(1)
+    G4_TILED_CASE(2)
+    G4_TILED_CASE(4)
+    G4_TILED_CASE(6)
+    G4_TILED_CASE(8)
+    G4_TILED_CASE(16)
+    default:
+      return false;
+  }
+#undef G4_TILED_CASE
+}
+
+template bool launch_g4_tiled<double>(TiledArgs<double>, hipStream_t);
+template bool launch_g4_tiled<float>(TiledArgs<float>, hipStream_t);
+
+}  // namespace g4acc

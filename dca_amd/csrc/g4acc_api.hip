@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -48,6 +49,10 @@ struct g4acc_handle {
   std::vector<void*> d_g4;         // per channel, 16 B per entry
   int *d_perm = nullptr, *d_off = nullptr, *d_err = nullptr;
   void *d_T = nullptr, *d_B = nullptr, *d_A = nullptr, *d_E = nullptr, *d_G = nullptr;
+  void* d_UV = nullptr;            // rank-one factors of the tiled single-band G4 kernel
+  bool fused_mtog = false;         // register-fused single-band M -> G (no E workspace)
+  bool use_tiled = false;
+  int tile_order = 0;              // G4 tile schedule (G4ACC_TILE_ORDER)
   SlotDesc* d_desc = nullptr;
   SlotDesc* h_desc = nullptr;      // pinned
   double* d_stage_f = nullptr;     // device staging of host inputs (M, tau)
@@ -232,6 +237,7 @@ MtoGArgs<Real> mtog_args(g4acc_handle* h, int n_slots) {
   a.n_slots = n_slots;
   a.n_max = h->max_vertices;
   a.chunk = h->chunk;
+  a.fused = h->fused_mtog;
   a.perm = h->d_perm;
   a.off = h->d_off;
   a.T = static_cast<Cplx<Real>*>(h->d_T);
@@ -253,6 +259,32 @@ void run_mtog(g4acc_handle* h, int n_slots) {
 
 template <typename Real>
 void run_g4(g4acc_handle* h, int n_walkers) {
+  if (h->use_tiled) {
+    TiledArgs<Real> a;
+    a.g = h->g;
+    a.G = static_cast<const Cplx<Real>*>(h->d_G);
+    a.UV = static_cast<Cplx<Real>*>(h->d_UV);
+    a.desc = h->d_desc;
+    a.n_walkers = n_walkers;
+    a.n_ch = static_cast<int>(h->channels.size());
+    a.need_y = a.need_z = a.has_trans = false;
+    for (int c = 0; c < a.n_ch; ++c) {
+      a.ch[c] = h->channels[c];
+      a.g4[c] = h->d_g4[c];
+      a.need_y |= a.ch[c] != G4ACC_PARTICLE_PARTICLE_UP_DOWN;
+      a.need_z |= a.ch[c] == G4ACC_PARTICLE_PARTICLE_UP_DOWN;
+      a.has_trans |= a.ch[c] == G4ACC_PARTICLE_HOLE_TRANSVERSE;
+    }
+    a.k_add = h->d_kadd;
+    a.k_sub = h->d_ksub;
+    a.k_ex = h->d_kex;
+    a.w_ex = h->d_wex;
+    a.nqg = 1;
+    a.tile_order = h->tile_order;
+    a.sum_mode = h->sum_mode;
+    if (launch_g4_tiled<Real>(a, h->stream))
+      return;
+  }
   for (size_t c = 0; c < h->channels.size(); ++c) {
     G4Args<Real> a;
     a.g = h->g;
@@ -391,8 +423,10 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
   const int nc2 = g.nc * g.nc;
   const size_t D = g.D, W2 = 2 * static_cast<size_t>(h->max_walkers), NM = h->max_vertices;
   const size_t cb = h->cplx_bytes();
-  // Slots whose S1..S4 workspaces (B, A: D x N; E: D x D) exist at once.
-  const size_t per_slot = cb * (2 * D * NM + D * D);
+  // Slots whose S1..S4 workspaces (B, A: D x N; E: D x D) exist at once. The fused single-band
+  // kernels need only A.
+  h->fused_mtog = mtog_fused_covers(g) && getenv("G4ACC_UNFUSED_MTOG") == nullptr;
+  const size_t per_slot = h->fused_mtog ? cb * D * NM : cb * (2 * D * NM + D * D);
   h->chunk = static_cast<int>(std::max<size_t>(1, std::min<size_t>(W2, workspace_budget_bytes() / per_slot)));
   const size_t CH = h->chunk;
 
@@ -403,14 +437,35 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
       (rc = h->alloc(&h->d_perm, W2 * NM, "perm")) ||
       (rc = h->alloc(&h->d_off, W2 * (g.n_orb + 1), "off")) || (rc = h->alloc(&h->d_err, 1, "err")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_T), cb * W2 * NM * g.nwE, "T")) ||
-      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_B), cb * CH * NM * D, "B")) ||
+      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_B), h->fused_mtog ? 1 : cb * CH * NM * D, "B")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_A), cb * CH * NM * D, "A")) ||
-      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_E), cb * CH * D * D, "E")) ||
+      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_E), h->fused_mtog ? 1 : cb * CH * D * D, "E")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_G), cb * W2 * D * D, "G")) ||
       (rc = h->alloc(&h->d_desc, W2, "desc")))
     return bail(rc);
   if (gm->band_phase && (rc = h->alloc(&h->d_bphase, static_cast<size_t>(g.nb) * g.nc, "band_phase")))
     return bail(rc);
+  // The LDS-tiled G4 kernel covers one band, Nc >= 8 and tiles whose staged G blocks fit the
+  // workgroup's LDS.
+  {
+    bool need_y = false, need_z = false;
+    for (int c : h->channels) {
+      need_y |= c != G4ACC_PARTICLE_PARTICLE_UP_DOWN;
+      need_z |= c == G4ACC_PARTICLE_PARTICLE_UP_DOWN;
+    }
+    int max_lds = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess)
+      max_lds = 65536;
+    const size_t lds = tiled_lds_bytes(g, need_y, need_z, cb) + cb * 256;
+    if (const char* to = getenv("G4ACC_TILE_ORDER"))
+      h->tile_order = atoi(to);
+    h->use_tiled = g.nb == 1 && g.nc >= 8 && tiled_pairs_per_thread(g.nc) > 0 &&
+                   lds <= static_cast<size_t>(max_lds) && getenv("G4ACC_GENERIC_G4") == nullptr;
+    if (h->use_tiled &&
+        (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_UV),
+                       cb * W2 / 2 * 4 * g.nq * g.nv * g.nc * g.nw, "UV")))
+      return bail(rc);
+  }
   h->d_g4.assign(h->channels.size(), nullptr);
   for (auto& p : h->d_g4)
     if ((rc = h->alloc(reinterpret_cast<double2**>(&p), h->g4_entries, "G4")))

@@ -12,6 +12,7 @@ struct MtoGArgs {
   int n_slots;            // 2 * walkers
   int n_max;              // vertex capacity per slot
   int chunk;              // slots whose S1..S4 workspaces exist at once
+  bool fused;             // use the register-fused single-band kernels
   const int* perm;        // [n_slots][n_max]
   const int* off;         // [n_slots][n_orb + 1]
   Cplx<Real>* T;          // [n_slots][n_max][nwE]
@@ -29,6 +30,10 @@ void launch_sort(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, int
 
 template <typename Real>
 void launch_mtog(const MtoGArgs<Real>& a, hipStream_t st);
+// Register-fused single-band M -> G (mtog_fused.hip); false if the geometry is not covered.
+template <typename Real>
+bool launch_mtog_fused(const MtoGArgs<Real>& a, hipStream_t st);
+bool mtog_fused_covers(const Geo& g);
 
 enum SumMode { SUM_FP64 = 0, SUM_FP32_KAHAN = 1 };
 
@@ -49,6 +54,34 @@ struct G4Args {
 
 template <typename Real>
 void launch_g4(const G4Args<Real>& a, hipStream_t st);
+
+// Single-band, all channels in one launch (g4_tiled.hip). Returns false when the geometry is
+// outside what the tiled kernel handles (the caller then uses launch_g4 per channel).
+template <typename Real>
+struct TiledArgs {
+  Geo g;
+  const Cplx<Real>* G;         // [n_slots][D][D]
+  Cplx<Real>* UV;              // [walkers][4][nq*nv][Nc*nw]
+  const SlotDesc* desc;
+  int n_walkers;
+  int n_ch;
+  int ch[4];                   // channel types in G4 order
+  void* g4[4];
+  const int* k_add;
+  const int* k_sub;
+  const int* k_ex;
+  const int* w_ex;
+  int nqg;                     // momentum groups per exchange frequency (set by the launcher)
+  int tile_order;              // 0: natural, 1: XCD-aware super-tiles
+  int super_tile, n_super, n_xcd;  // tile schedule (set by the launcher)
+  int sum_mode;
+  bool need_y, need_z, has_trans;
+};
+
+template <typename Real>
+bool launch_g4_tiled(TiledArgs<Real> a, hipStream_t st);
+int tiled_pairs_per_thread(int nc);
+size_t tiled_lds_bytes(const Geo& g, bool need_y, bool need_z, size_t cplx_bytes);
 
 // dst += src over n complex entries of a G4 buffer in the given sum mode.
 void launch_g4_add(void* dst, const void* src, size_t n, int sum_mode, hipStream_t st);

@@ -1,0 +1,207 @@
+// Single-band M -> G with register-resident intermediates (n_bands == 1, Nc a template constant).
+//
+// Same contractions as mtog_kernels.hip (S1..S4), fused so that the orbital-resolved
+// intermediates never reach memory:
+//   left  (per slot, thread = (w, j)):
+//     B[r]          = sum_{i in site r} T[w, i] M[perm i, perm j]          r = 0..Nc-1 in registers
+//     A[(k, w), j]  = sum_r dft[k, r] B[r]                                 written once
+//   right (per slot, thread = (p1, a group of TW frequencies w2)):
+//     E[t][r]       = sum_{j in site r} A[p1, j] conj(T[w2_t, j])          in registers
+//     M~(p1,(k2,w2))= (1/Nc) sum_r E[t][r] conj(dft[k2, r]) x band phases
+//     G(p1,p2)      = -G0(p1) M~ G0(p2) + delta_{p1 p2} beta G0(p1)        FP64 epilogue, stored once
+// T[w, j] = exp(i w_w tau_{perm j}) and the site runs of the sorted vertices come from
+// sort_kernel / phase_kernel. Per slot this is 8 nwE N^2 + 8 Nc D N + 8 D N nwE + 8 Nc D^2 flops
+// and one write of G: the E matrix (D x D) of the unfused path is never stored.
+#include "g4acc_common.hpp"
+#include "g4acc_kernels.hpp"
+
+namespace g4acc {
+
+template <typename Real, int NC>
+__global__ void __launch_bounds__(256) mtog_left_fused(const SlotDesc* __restrict__ desc, Geo g,
+                                                       int n_max, int slot0,
+                                                       const int* __restrict__ perm,
+                                                       const int* __restrict__ off,
+                                                       const Cplx<Real>* __restrict__ T,
+                                                       const Cplx<Real>* __restrict__ dft,
+                                                       Cplx<Real>* __restrict__ A) {
+  using C = Cplx<Real>;
+  const int slot = slot0 + blockIdx.z;
+  const SlotDesc d = desc[slot];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = t % g.nwE, jj = t / g.nwE;
+  if (jj >= d.n)
+    return;
+  const int* P = perm + static_cast<size_t>(slot) * n_max;
+  const int* O = off + static_cast<size_t>(slot) * (NC + 1);
+  const C* Ts = T + static_cast<size_t>(slot) * g.nwE * n_max;
+  const double* mcol = d.m + static_cast<size_t>(d.ld) * P[jj];
+  C b[NC];
+#pragma unroll
+  for (int r = 0; r < NC; ++r) {
+    C acc{0, 0};
+    for (int ii = O[r]; ii < O[r + 1]; ++ii) {
+      const Real m = static_cast<Real>(mcol[P[ii]]);
+      const C tw = Ts[w + static_cast<size_t>(g.nwE) * ii];
+      acc.x = fma(tw.x, m, acc.x);
+      acc.y = fma(tw.y, m, acc.y);
+    }
+    b[r] = acc;
+  }
+  C* Ac = A + static_cast<size_t>(blockIdx.z) * g.D * n_max + static_cast<size_t>(g.D) * jj + NC * w;
+#pragma unroll 4
+  for (int k = 0; k < NC; ++k) {
+    C acc{0, 0};
+#pragma unroll
+    for (int r = 0; r < NC; ++r)
+      cfma(acc, dft[k + NC * r], b[r]);
+    Ac[k] = acc;
+  }
+}
+
+template <typename Real, int NC, int TW>
+__global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restrict__ desc, Geo g,
+                                                        int n_max, int slot0,
+                                                        const int* __restrict__ off,
+                                                        const Cplx<Real>* __restrict__ T,
+                                                        const Cplx<Real>* __restrict__ A,
+                                                        const Cplx<Real>* __restrict__ dft,
+                                                        const double2* __restrict__ bphase,
+                                                        const double2* __restrict__ G0,
+                                                        Cplx<Real>* __restrict__ G) {
+  using C = Cplx<Real>;
+  const int slot = slot0 + blockIdx.z;
+  const int s = slot & 1;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int p1 = blockIdx.x * 64 + lane;
+  const int w2b = (blockIdx.y * 4 + wave) * TW;
+  if (w2b >= g.nwE)
+    return;
+  if (p1 >= g.D)
+    return;
+  const int pr = p1;
+  const int* O = off + static_cast<size_t>(slot) * (NC + 1);
+  const C* Ts = T + static_cast<size_t>(slot) * g.nwE * n_max;
+  const C* As = A + static_cast<size_t>(blockIdx.z) * g.D * n_max + pr;
+
+  C e[TW][NC];
+#pragma unroll
+  for (int tt = 0; tt < TW; ++tt)
+#pragma unroll
+    for (int r = 0; r < NC; ++r)
+      e[tt][r] = C{0, 0};
+  const int tw_ok = g.nwE - w2b;  // frequencies of this group inside the domain
+#pragma unroll
+  for (int r = 0; r < NC; ++r) {
+    const int j0 = O[r], j1 = O[r + 1];
+    int jj = j0;
+    // Four independent row loads in flight per step.
+    for (; jj + 4 <= j1; jj += 4) {
+      C av[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        av[u] = As[static_cast<size_t>(g.D) * (jj + u)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const C* tcol = Ts + static_cast<size_t>(g.nwE) * (jj + u) + w2b;
+#pragma unroll
+        for (int tt = 0; tt < TW; ++tt)
+          if (tt < tw_ok)
+            cfmac(e[tt][r], av[u], tcol[tt]);
+      }
+    }
+    for (; jj < j1; ++jj) {
+      const C a = As[static_cast<size_t>(g.D) * jj];
+      const C* tcol = Ts + static_cast<size_t>(g.nwE) * jj + w2b;
+#pragma unroll
+      for (int tt = 0; tt < TW; ++tt)
+        if (tt < tw_ok)
+          cfmac(e[tt][r], a, tcol[tt]);
+    }
+  }
+  // Epilogue: right site DFT, band phases, G = -G0 M~ G0 + beta G0 delta (FP64).
+  const int k1 = p1 % NC, w1 = p1 / NC;
+  auto g0 = [&](int k, int w) { return G0[s + 2 * (k + static_cast<size_t>(NC) * w)]; };
+  const double2 ga = g0(k1, w1);
+  const double inv_nc = 1.0 / NC;
+  C* Gs = G + static_cast<size_t>(slot) * g.D * g.D + p1;
+#pragma unroll
+  for (int tt = 0; tt < TW; ++tt) {
+    const int w2 = w2b + tt;
+    if (w2 >= g.nwE)
+      break;
+#pragma unroll 2
+    for (int k2 = 0; k2 < NC; ++k2) {
+      C acc{0, 0};
+#pragma unroll
+      for (int r = 0; r < NC; ++r)
+        cfmac(acc, e[tt][r], dft[k2 + NC * r]);
+      double2 m = to_d(acc);
+      m.x *= inv_nc;
+      m.y *= inv_nc;
+      if (bphase)
+        m = cmulc(cmul(m, bphase[k1]), bphase[k2]);
+      double2 v = cmul(cmul(ga, m), g0(k2, w2));
+      v.x = -v.x;
+      v.y = -v.y;
+      const int p2 = k2 + NC * w2;
+      if (p1 == p2) {
+        v.x = fma(ga.x, g.beta, v.x);
+        v.y = fma(ga.y, g.beta, v.y);
+      }
+      Gs[static_cast<size_t>(g.D) * p2] = from_d<C>(v);
+    }
+  }
+}
+
+namespace {
+
+template <typename Real, int NC, int TW>
+void launch_fused_nc(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st) {
+  const Geo& g = a.g;
+  hipLaunchKernelGGL((mtog_left_fused<Real, NC>), dim3((g.nwE * a.n_max + 255) / 256, 1, ns), dim3(256), 0,
+                     st, a.desc, g, a.n_max, s0, a.perm, a.off, a.T, a.dft, a.A);
+  hipLaunchKernelGGL((mtog_right_fused<Real, NC, TW>), dim3((g.D + 63) / 64, (g.nwE + 4 * TW - 1) / (4 * TW), ns),
+                     dim3(256), 0, st, a.desc, g, a.n_max, s0, a.off, a.T, a.A, a.dft, a.bphase, a.G0, a.G);
+}
+
+}  // namespace
+
+template <typename Real>
+bool launch_mtog_fused(const MtoGArgs<Real>& a, hipStream_t st) {
+  const Geo& g = a.g;
+  if (g.nb != 1)
+    return false;
+  constexpr bool dp = sizeof(Real) == 8;
+  auto run = [&](auto launcher) {
+    for (int s0 = 0; s0 < a.n_slots; s0 += a.chunk) {
+      const int ns = (a.n_slots - s0) < a.chunk ? (a.n_slots - s0) : a.chunk;
+      launcher(s0, ns);
+    }
+    return true;
+  };
+  switch (g.nc) {
+    case 4:
+      return run([&](int s0, int ns) { launch_fused_nc<Real, 4, 8>(a, s0, ns, st); });
+    case 8:
+      return run([&](int s0, int ns) { launch_fused_nc<Real, 8, (dp ? 4 : 8)>(a, s0, ns, st); });
+    case 16:
+      return run([&](int s0, int ns) { launch_fused_nc<Real, 16, (dp ? 2 : 4)>(a, s0, ns, st); });
+    case 36:
+      return run([&](int s0, int ns) { launch_fused_nc<Real, 36, 1>(a, s0, ns, st); });
+    case 64:
+      return run([&](int s0, int ns) { launch_fused_nc<Real, 64, 1>(a, s0, ns, st); });
+    default:
+      return false;
+  }
+}
+
+bool mtog_fused_covers(const Geo& g) {
+  return g.nb == 1 && (g.nc == 4 || g.nc == 8 || g.nc == 16 || g.nc == 36 || g.nc == 64);
+}
+
+template bool launch_mtog_fused<double>(const MtoGArgs<double>&, hipStream_t);
+template bool launch_mtog_fused<float>(const MtoGArgs<float>&, hipStream_t);
+
+}  // namespace g4acc

@@ -251,6 +251,8 @@ void launch_mtog(const MtoGArgs<Real>& a, hipStream_t st) {
   const Geo& g = a.g;
   hipLaunchKernelGGL(phase_kernel<Real>, dim3((g.nwE * a.n_max + 255) / 256, a.n_slots), dim3(256), 0, st,
                      a.desc, g, a.n_max, a.perm, a.T);
+  if (a.fused && launch_mtog_fused<Real>(a, st))
+    return;
   for (int s0 = 0; s0 < a.n_slots; s0 += a.chunk) {
     const int ns = (a.n_slots - s0) < a.chunk ? (a.n_slots - s0) : a.chunk;
     hipLaunchKernelGGL(ndft_left_kernel<Real>, dim3((g.D + 255) / 256, a.n_max, ns), dim3(256), 0, st,

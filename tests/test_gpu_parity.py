@@ -26,6 +26,12 @@ CASES = {
     "1band_2x2_negative_wex": (1, 2, 4, (-2,), ((math.pi, 0.0),), (1, 8)),
     "3band_2x2": (3, 2, 4, (0, 1), ((0.0, 0.0), (math.pi, math.pi)), (2, 10)),
     "2band_8site": (2, "8site", 4, (0, 1), ((0.0, 0.0), (math.pi, math.pi)), (2, 12)),
+    # single band with Nc >= 8: the LDS-tiled G4 kernel
+    "1band_4x4_4q": (1, 4, 4, (0, 1, 2), ((0.0, 0.0), (math.pi, 0.0), (math.pi, math.pi),
+                                           (math.pi / 2, math.pi / 2)), (10, 40)),
+    "1band_8site": (1, "8site", 6, (0, 1), ((0.0, 0.0), (math.pi, math.pi)), (4, 20)),
+    "1band_6x6": (1, 6, 2, (0, 1), ((0.0, 0.0), (math.pi, 0.0), (math.pi, math.pi),
+                                    (2 * math.pi / 3, 0.0)), (10, 60)),
 }
 
 
@@ -183,3 +189,38 @@ def test_sum_to_and_errors(oracle_lib):
     bad.site[0][0] = g.nc  # out of range
     with pytest.raises(g4acc.LogicError, match="out of range"):
         b.accumulate(bad)
+
+
+@pytest.mark.parametrize("name", ["1band_4x4_4q", "1band_6x6", "1band_8site"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_tiled_equals_generic_kernel(name, precision, monkeypatch):
+    """The LDS-tiled single-band G4 kernel and the generic per-entry kernel agree."""
+    wl = case(name, channels=[geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE,
+                              geo.PARTICLE_PARTICLE_UP_DOWN])
+    g = wl.geometry
+    ms = wl.draw(5)
+    tiled = g4acc.TpAccumulator(g, precision=precision, max_walkers=5, max_vertices=wl.n_max)
+    monkeypatch.setenv("G4ACC_GENERIC_G4", "1")
+    generic = g4acc.TpAccumulator(g, precision=precision, max_walkers=5, max_vertices=wl.n_max)
+    for acc in (tiled, generic):
+        acc.resetAccumulation()
+        acc.accumulate_batch(ms)
+    for a, b in zip(tiled.get_G4(), generic.get_G4()):
+        err = linf(b, a)
+        print(f"{name} {precision} tiled vs generic l_inf={err:.3e}", flush=True)
+        assert err < (1e-14 if precision == "fp64" else 1e-7)
+
+
+@pytest.mark.parametrize("name", ["1band_2x2", "1band_4x4_4q", "1band_6x6", "1band_8site"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_equals_unfused_mtog(name, precision, monkeypatch):
+    """The register-fused single-band M -> G kernels and the four-stage kernels agree."""
+    wl = case(name)
+    fused = g4acc.TpAccumulator(wl.geometry, precision=precision, max_vertices=wl.n_max)
+    monkeypatch.setenv("G4ACC_UNFUSED_MTOG", "1")
+    unfused = g4acc.TpAccumulator(wl.geometry, precision=precision, max_vertices=wl.n_max)
+    for meas in wl.draw(2):
+        a, b = fused.compute_G(meas), unfused.compute_G(meas)
+        err = linf(b, a)
+        print(f"{name} {precision} fused vs unfused G l_inf={err:.3e}", flush=True)
+        assert err < (1e-14 if precision == "fp64" else 1e-6)
