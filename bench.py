@@ -95,10 +95,11 @@ class DevicePool:
 
 
 def run_batches(acc, batches, steps):
+    from dca_amd import g4acc
     lib, h = acc.lib, acc._h
     for k in range(steps):
         arr, n = batches[k % len(batches)]
-        rc = lib.g4acc_accumulate_batch(h, arr, n, 1)
+        rc = lib.g4acc_accumulate_batch(h, arr, n, g4acc.DEVICE)
         if rc != 0:
             acc._check(rc)
 
@@ -222,7 +223,7 @@ def main():
                                       max_vertices=wl.n_max)
             acc.resetAccumulation()
             arr = (g4acc.Walker_c * S)(*dp.walkers[:S])
-            acc._check(acc.lib.g4acc_accumulate_batch(acc._h, arr, S, 1))
+            acc._check(acc.lib.g4acc_accumulate_batch(acc._h, arr, S, g4acc.DEVICE))
             got = acc.get_G4()
             result_err[name] = max(bf.difference(r, x)["l_inf"] for r, x in zip(g4_ref, got))
             acc.close()

@@ -52,7 +52,7 @@ struct g4acc_handle {
   void* d_UV = nullptr;            // rank-one factors of the tiled single-band G4 kernel
   bool fused_mtog = false;         // register-fused single-band M -> G (no E workspace)
   bool use_tiled = false;
-  int tile_order = 0;              // G4 tile schedule (G4ACC_TILE_ORDER)
+  int tile_order = -1;             // G4 tile schedule (G4ACC_TILE_ORDER); -1 = by batch size
   SlotDesc* d_desc = nullptr;
   SlotDesc* h_desc = nullptr;      // pinned
   double* d_stage_f = nullptr;     // device staging of host inputs (M, tau)
@@ -184,12 +184,21 @@ int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int m
         d.ld = 1;
         continue;
       }
-      if (memory == G4ACC_HOST) {
+      if (!(memory & 1)) {  // M in host memory: packed column by column
         double* fm = h->h_stage_f + of;
         for (int j = 0; j < n; ++j)
           std::memcpy(fm + static_cast<size_t>(n) * j, wk.m[s] + static_cast<size_t>(wk.ld[s]) * j,
                       sizeof(double) * n);
-        std::memcpy(fm + static_cast<size_t>(n) * n, wk.tau[s], sizeof(double) * n);
+        d.m = h->d_stage_f + of;
+        d.ld = n;
+        of += static_cast<size_t>(n) * n;
+      }
+      else {
+        d.m = wk.m[s];
+        d.ld = wk.ld[s];
+      }
+      if (!(memory & 2)) {  // configuration in host memory: validated and packed
+        std::memcpy(h->h_stage_f + of, wk.tau[s], sizeof(double) * n);
         int* im = h->h_stage_i + oi;
         for (int i = 0; i < n; ++i) {
           const int r = wk.site[s][i], b = wk.band[s][i];
@@ -198,30 +207,26 @@ int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int m
           im[i] = r;
           im[n + i] = b;
         }
-        d.m = h->d_stage_f + of;
-        d.tau = h->d_stage_f + of + static_cast<size_t>(n) * n;
+        d.tau = h->d_stage_f + of;
         d.site = h->d_stage_i + oi;
         d.band = h->d_stage_i + oi + n;
-        d.ld = n;
-        of += static_cast<size_t>(n) * n + n;
+        of += n;
         oi += 2 * static_cast<size_t>(n);
       }
       else {
-        d.m = wk.m[s];
         d.tau = wk.tau[s];
         d.site = wk.site[s];
         d.band = wk.band[s];
-        d.ld = wk.ld[s];
       }
     }
   }
   const int n_slots = 2 * nw_;
-  if (memory == G4ACC_HOST && of) {
+  if (of)
     G4_TRY(h->check(hipMemcpyAsync(h->d_stage_f, h->h_stage_f, of * sizeof(double),
-                                   hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync(M)"));
+                                   hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync(M, tau)"));
+  if (oi)
     G4_TRY(h->check(hipMemcpyAsync(h->d_stage_i, h->h_stage_i, oi * sizeof(int),
                                    hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync(config)"));
-  }
   G4_TRY(h->check(hipMemcpyAsync(h->d_desc, h->h_desc, sizeof(SlotDesc) * n_slots,
                                  hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync(desc)"));
   G4_TRY(h->check(hipEventRecord(h->staged, h->stream), "hipEventRecord"));
@@ -280,7 +285,9 @@ void run_g4(g4acc_handle* h, int n_walkers) {
     a.k_ex = h->d_kex;
     a.w_ex = h->d_wex;
     a.nqg = 1;
-    a.tile_order = h->tile_order;
+    // XCD-aware super-tiles pay off when many walkers re-read G; a single walker's launch is
+    // dominated by the G4 read-modify-write, which the natural order streams best.
+    a.tile_order = h->tile_order >= 0 ? h->tile_order : (n_walkers > 1 ? 1 : 0);
     a.sum_mode = h->sum_mode;
     if (launch_g4_tiled<Real>(a, h->stream))
       return;
@@ -552,7 +559,8 @@ int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_w
     return G4ACC_INVALID_ARG;
   if (!h->initialized)
     return h->fail(G4ACC_NOT_INITIALIZED, "accumulate called before resetAccumulation");
-  if (n_walkers < 0 || (n_walkers > 0 && !walkers) || (memory != G4ACC_HOST && memory != G4ACC_DEVICE))
+  if (n_walkers < 0 || (n_walkers > 0 && !walkers) ||
+      (memory != G4ACC_HOST && memory != G4ACC_DEVICE_M && memory != G4ACC_DEVICE))
     return h->fail(G4ACC_INVALID_ARG, "invalid batch arguments");
   if (!h->g4_present)
     return G4ACC_OK;  // after sumTo the reference's G4 vector is empty: nothing is accumulated
@@ -651,6 +659,8 @@ int g4acc_compute_G(g4acc_handle* h, const g4acc_walker* walker, int memory, dou
     return G4ACC_INVALID_ARG;
   if (n_complex < g4acc_g_size(h))
     return h->fail(G4ACC_INVALID_ARG, "output buffer too small");
+  if (memory != G4ACC_HOST && memory != G4ACC_DEVICE_M && memory != G4ACC_DEVICE)
+    return h->fail(G4ACC_INVALID_ARG, "invalid memory kind");
   G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
   G4_TRY(check_walker(h, *walker));
   const g4acc_walker* ws[1] = {walker};

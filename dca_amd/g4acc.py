@@ -25,7 +25,7 @@ LIB_PATH = os.path.join(HERE, "libg4acc.so")
 OK, INVALID_ARG, UNSUPPORTED_CHANNEL, NOT_INITIALIZED, OUT_OF_MEMORY, DEVICE_ERROR, EMPTY = range(7)
 FP64, FP32 = 0, 1
 SUM_FP64, SUM_FP32_KAHAN = 0, 1
-HOST, DEVICE = 0, 1
+HOST, DEVICE_M, DEVICE = 0, 1, 3
 
 EXPORTED = [
     "g4acc_create", "g4acc_destroy", "g4acc_reset", "g4acc_accumulate", "g4acc_accumulate_batch",

@@ -42,8 +42,9 @@ typedef enum g4acc_channel {
 /* Arithmetic of the M -> G step. The reference's choice is TpAccumulationPrecision
  * (include/dca/config/mc_options.hpp.in:28, float or double). */
 typedef enum g4acc_precision {
-  G4ACC_FP64 = 0,   /* complex128 G, FP64 products (MFMA f64) */
-  G4ACC_FP32 = 1    /* complex64 G, FP32 products with FP32 accumulation (MFMA f32) */
+  G4ACC_FP64 = 0,   /* complex128 G, FP64 products and sums */
+  G4ACC_FP32 = 1    /* complex64 G, FP32 products with FP32 accumulation; phases, the G epilogue
+                       and the G4 sums stay FP64 */
 } g4acc_precision;
 
 /* Running sum of G4 over measurements. */
@@ -52,9 +53,14 @@ typedef enum g4acc_sum_mode {
   G4ACC_SUM_FP32_KAHAN = 1   /* FP32 G4 with a per-entry FP32 Kahan compensation term */
 } g4acc_sum_mode;
 
+/* Where a walker's arrays live. Host arrays are copied into the library's pinned staging buffer
+ * before the call returns; device arrays are read asynchronously until g4acc_sync. */
 typedef enum g4acc_memory {
-  G4ACC_HOST = 0,    /* walker arrays are host memory (copied to the device by the call) */
-  G4ACC_DEVICE = 1   /* walker arrays are device memory of the accumulator's device */
+  G4ACC_HOST = 0,      /* M and the configuration (tau, site, band) in host memory */
+  G4ACC_DEVICE_M = 1,  /* M in device memory, configuration in host memory (a GPU walker's
+                          computeM output with the host-side configuration, as the reference's
+                          GPU accumulator takes them, tp_accumulator_gpu.hpp:321-355) */
+  G4ACC_DEVICE = 3     /* M and the configuration in device memory */
 } g4acc_memory;
 
 /* Everything the reference constructor reads from Parameters, the static domains and G0

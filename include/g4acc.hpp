@@ -1,0 +1,188 @@
+// C++ host mirror of DCA++'s TpAccumulator over the g4acc C ABI (g4acc.h).
+//
+// Drop-in for accumulator::TpAccumulator<Parameters, DistType::NONE, DeviceType> as the CT-AUX
+// and CT-INT accumulators use it (ctaux_accumulator.hpp:69,278,303,339,394,522,544;
+// ctint_accumulator.hpp:155,175,193,245,258,270,289): same method names, the same argument
+// meaning, and the reference's exception types (std::logic_error, std::bad_alloc,
+// std::runtime_error) rethrown from the ABI's status codes.
+//
+// Template arguments are duck-typed on the reference's types:
+//   Matrix         nrRows(), leadingDimension(), ptr()           (dca::linalg::Matrix<Real, ...>)
+//   Configuration  size(), operator[](i).get_tau(),
+//                  operator[](i).get_right_band(), .get_right_site()   (CT-AUX vertex_singleton,
+//                  ctaux/structs/vertex_singleton.hpp:84-95; CT-INT sector entries)
+// The orbital of a vertex is taken from its right band and site, as CachedNdftBase does for
+// density-density interactions (tp/ndft/cached_ndft_base.hpp:97-103).
+#ifndef G4ACC_HPP
+#define G4ACC_HPP
+
+#include <algorithm>
+#include <array>
+#include <complex>
+#include <cstddef>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "g4acc.h"
+
+namespace g4acc {
+
+inline void throw_on_error(int status, const char* message) {
+  switch (status) {
+    case G4ACC_OK:
+      return;
+    case G4ACC_OUT_OF_MEMORY:
+      throw std::bad_alloc();
+    case G4ACC_DEVICE_ERROR:
+      throw std::runtime_error(message ? message : "g4acc device error");
+    default:
+      throw std::logic_error(message ? message : "g4acc error");
+  }
+}
+
+// Owning version of g4acc_geometry (everything the reference constructor reads from
+// Parameters, the static domains and G0).
+struct Geometry {
+  int n_bands = 1;
+  int n_sites = 1;
+  double beta = 1.;
+  int n_freq = 2;                         // 2 * four-point-fermionic-frequencies
+  std::vector<int> w_ex{0};               // FrequencyExchangeDomain::get_elements()
+  std::vector<int> k_ex{0};               // MomentumExchangeDomain::get_elements()
+  std::vector<int> channels;              // sorted, unique FourPointType values
+  std::vector<int> k_add, k_sub;          // [i + Nc j]
+  std::vector<std::complex<double>> dft;  // [k + Nc r] = exp(i k.r)
+  std::vector<std::complex<double>> band_phase;  // [b + nb k], empty: no band displacement
+  std::vector<std::complex<double>> g0;   // (b1, b2, s, k, wE)
+
+  g4acc_geometry view() const {
+    g4acc_geometry g{};
+    g.n_bands = n_bands;
+    g.n_sites = n_sites;
+    g.beta = beta;
+    g.n_freq = n_freq;
+    g.n_w_ex = static_cast<int>(w_ex.size());
+    g.w_ex = w_ex.data();
+    g.n_k_ex = static_cast<int>(k_ex.size());
+    g.k_ex = k_ex.data();
+    g.n_channels = static_cast<int>(channels.size());
+    g.channels = channels.data();
+    g.k_add = k_add.data();
+    g.k_sub = k_sub.data();
+    g.dft = reinterpret_cast<const double*>(dft.data());
+    g.band_phase = band_phase.empty() ? nullptr : reinterpret_cast<const double*>(band_phase.data());
+    g.g0 = reinterpret_cast<const double*>(g0.data());
+    return g;
+  }
+};
+
+class TpAccumulator {
+public:
+  using TpComplex = std::complex<double>;
+  using TpGreensFunction = std::vector<TpComplex>;  // reference order, first index fastest
+
+  TpAccumulator(const Geometry& geometry, const g4acc_options& options)
+      : geometry_(geometry) {
+    const g4acc_geometry g = geometry_.view();
+    throw_on_error(g4acc_create(&g, &options, &handle_), g4acc_last_error(nullptr));
+  }
+  ~TpAccumulator() {
+    if (handle_)
+      g4acc_destroy(handle_);
+  }
+  TpAccumulator(const TpAccumulator&) = delete;
+  TpAccumulator& operator=(const TpAccumulator&) = delete;
+  TpAccumulator(TpAccumulator&& other) noexcept
+      : geometry_(std::move(other.geometry_)), handle_(std::exchange(other.handle_, nullptr)) {}
+
+  // Zeroes G4 (tp_accumulator_cpu.hpp:224-229). The loop index is accepted for signature parity.
+  void resetAccumulation(unsigned int /*dca_loop*/ = 0) { check(g4acc_reset(handle_)); }
+
+  // accumulate(M_pair, configs, sign) (tp_accumulator_cpu.hpp:107-109). `memory` says where the
+  // M matrices live; the configuration is always read from the host objects. Returns the
+  // algorithmic flop count of the measurement.
+  template <class MatrixPair, class Configuration, class Sign>
+  double accumulate(const MatrixPair& M_pair, const std::array<Configuration, 2>& configs,
+                    const Sign sign, int memory = G4ACC_HOST) {
+    g4acc_walker w{};
+    for (int s = 0; s < 2; ++s) {
+      const auto& conf = configs[s];
+      const int n = static_cast<int>(conf.size());
+      tau_[s].resize(n);
+      site_[s].resize(n);
+      band_[s].resize(n);
+      for (int i = 0; i < n; ++i) {
+        tau_[s][i] = conf[i].get_tau();
+        band_[s][i] = conf[i].get_right_band();
+        site_[s][i] = conf[i].get_right_site();
+      }
+      w.n[s] = n;
+      w.m[s] = n ? M_pair[s].ptr() : nullptr;
+      w.ld[s] = n ? M_pair[s].leadingDimension() : 1;
+      w.tau[s] = tau_[s].data();
+      w.site[s] = site_[s].data();
+      w.band[s] = band_[s].data();
+    }
+    w.sign = static_cast<double>(sign);
+    const int kind = memory == G4ACC_HOST ? G4ACC_HOST : G4ACC_DEVICE_M;
+    check(g4acc_accumulate(handle_, &w, kind));
+    return flops(w.n[0], w.n[1]);
+  }
+
+  // Waits for queued work (tp_accumulator_gpu.hpp:471-484).
+  void finalize() { check(g4acc_finalize(handle_)); }
+  // After this returns the M matrices passed to accumulate may be overwritten
+  // (ctaux_accumulator.hpp:394).
+  void synchronizeCopy() { check(g4acc_sync(handle_)); }
+  template <class T>
+  void syncStreams(const T&) {
+    synchronizeCopy();
+  }
+
+  // get_G4() (tp_accumulator_cpu.hpp:871-877): one vector per channel, in the reference order.
+  const std::vector<TpGreensFunction>& get_G4() {
+    const std::size_t n = g4acc_g4_size(handle_);
+    G4_.assign(geometry_.channels.size(), TpGreensFunction(n));
+    for (std::size_t c = 0; c < G4_.size(); ++c)
+      check(g4acc_get_g4(handle_, static_cast<int>(c), reinterpret_cast<double*>(G4_[c].data()), n));
+    return G4_;
+  }
+
+  // sumTo(other) (tp_accumulator_cpu.hpp:895-904): adds into other and empties this G4.
+  void sumTo(TpAccumulator& other) { check(g4acc_sum_to(handle_, other.handle_)); }
+
+  void* get_stream() const { return g4acc_stream(handle_); }
+  std::size_t deviceFingerprint() const { return g4acc_bytes(handle_); }
+  g4acc_handle* handle() const { return handle_; }
+
+private:
+  void check(int status) const { throw_on_error(status, g4acc_last_error(handle_)); }
+
+  // M -> G: 8 nwE N^2 + 8 Nc D N + 8 D N nwE + 8 Nc D^2 per spin; G4: 8 flops per product term.
+  double flops(int n0, int n1) const {
+    int ext = 0;
+    for (int w : geometry_.w_ex)
+      ext = std::max(ext, w < 0 ? -w : w);
+    const double nwE = geometry_.n_freq + 2 * ext, nc = geometry_.n_sites, nb = geometry_.n_bands;
+    const double D = nb * nc * nwE;
+    double f = 0;
+    for (int n : {n0, n1})
+      f += 8 * nwE * n * n + 8 * nc * D * n + 8 * D * n * nwE + 8 * nc * D * D;
+    const double g4 = nb * nb * nb * nb * nc * nc * geometry_.n_freq * geometry_.n_freq *
+                      geometry_.k_ex.size() * geometry_.w_ex.size();
+    return f + 8. * 6. * g4 * geometry_.channels.size();
+  }
+
+  Geometry geometry_;
+  g4acc_handle* handle_ = nullptr;
+  std::array<std::vector<double>, 2> tau_;
+  std::array<std::vector<int>, 2> site_, band_;
+  std::vector<TpGreensFunction> G4_;
+};
+
+}  // namespace g4acc
+
+#endif  // G4ACC_HPP

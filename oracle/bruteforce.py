@@ -81,50 +81,41 @@ def logical_G(geom: geo.Geometry, G: np.ndarray) -> np.ndarray:
 
 def g4_free(geom: geo.Geometry, channel: int, sign: float) -> np.ndarray:
     """G4 of one measurement with M = 0 on both spins, from closed forms, shaped
-    (b1, b2, b3, b4, k1, w1, k2, w2, k_ex, w_ex) (Fortran order)."""
+    (b1, b2, b3, b4, k1, w1, k2, w2, k_ex, w_ex) (Fortran order). With M = 0,
+    G_s(b,b';k1,k2;w1,w2) = beta G0(b',b;k1,w1) delta_{k1k2} delta_{w1w2} (getGMultiband's
+    band-transposed read), G0 spin independent, so (a = sign/2, sum over both spins):
+      PP      :  sign beta^2 G0(b3,b1;k,w) G0(b4,b2;q-k,v-w)                    on k1=k2, w1=w2
+      MAG     : -sign beta^2 G0(b1,b3;k,w) G0(b4,b2;k+q,w+v)                    on k1=k2, w1=w2
+      CHARGE  : -sign beta^2 G0(b4,b1;k,w) G0(b3,b2;k+q,w+v)                    on k1=k2, w1=w2
+                + 2 sign beta^2 G0(b1,b2;k1,w1) G0(b4,b3;k2,w2)                 if q = 0, v = 0
+    """
     nb, nc, nw = geom.nb, geom.nc, geom.n_freq
     ext = geom.ext
     add, sub = geom.cluster.add, geom.cluster.sub
     g0 = geom.g0[:, :, 0]  # spin independent (b, b', k, wE)
-    beta = geom.beta
+    b2 = geom.beta ** 2
     out = np.zeros(geom.g4_shape(), complex)
     gamma = geom.cluster.k_index((0.0, 0.0))
+    ks = np.arange(nc)
+    ws = np.arange(nw)
+    g1 = g0[:, :, :, ext:ext + nw]  # (b, b', k, w) at the compact frequencies
     for iv, nu in enumerate(geom.w_ex):
         for iq, q in enumerate(geom.k_ex):
-            for k1 in range(nc):
-                for w1 in range(nw):
-                    e1 = w1 + ext
-                    for b1 in range(nb):
-                        for b2 in range(nb):
-                            for b3 in range(nb):
-                                for b4 in range(nb):
-                                    if channel == geo.PARTICLE_PARTICLE_UP_DOWN:
-                                        # sign beta^2 G0(b3,b1,k1,w1) G0(b4,b2,q-k1,nu-w1) delta
-                                        kq = sub[k1, q]
-                                        wq = nu + nw - 1 - w1 + ext
-                                        out[b1, b2, b3, b4, k1, w1, k1, w1, iq, iv] += \
-                                            sign * beta ** 2 * g0[b3, b1, k1, e1] * g0[b4, b2, kq, wq]
-                                    elif channel in (geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE):
-                                        kq = add[k1, q]
-                                        wq = w1 + nu + ext
-                                        if channel == geo.PARTICLE_HOLE_MAGNETIC:
-                                            # -alpha sum_s L(b3,b1;k1,k1;w1,w1) L(b2,b4;k1+q,k1+q;w1+nu,w1+nu)
-                                            val = g0[b1, b3, k1, e1] * g0[b4, b2, kq, wq]
-                                        else:
-                                            # -alpha sum_s L(b1,b4;...) L(b2,b3;...)
-                                            val = g0[b4, b1, k1, e1] * g0[b3, b2, kq, wq]
-                                        out[b1, b2, b3, b4, k1, w1, k1, w1, iq, iv] -= sign * beta ** 2 * val
+            if channel == geo.PARTICLE_PARTICLE_UP_DOWN:
+                g2 = g0[:, :, sub[ks, q]][:, :, :, nu + nw - 1 - ws + ext]   # (b, b', k, w)
+                val = sign * b2 * np.einsum("cakw,dbkw->abcdkw", g1, g2)
+            elif channel == geo.PARTICLE_HOLE_MAGNETIC:
+                g2 = g0[:, :, add[ks, q]][:, :, :, ws + nu + ext]
+                val = -sign * b2 * np.einsum("ackw,dbkw->abcdkw", g1, g2)
+            elif channel == geo.PARTICLE_HOLE_CHARGE:
+                g2 = g0[:, :, add[ks, q]][:, :, :, ws + nu + ext]
+                val = -sign * b2 * np.einsum("dakw,cbkw->abcdkw", g1, g2)
+            else:
+                raise ValueError("no closed form for this channel")
+            for k in range(nc):
+                out[:, :, :, :, k, :, k, :, iq, iv][..., ws, ws] += val[..., k, :]
             if channel == geo.PARTICLE_HOLE_CHARGE and q == gamma and nu == 0:
-                # 2 sign beta^2 L(b2,b1;k1,k1;w1,w1) L(b3,b4;k2,k2;w2,w2)
-                for k1 in range(nc):
-                    for w1 in range(nw):
-                        for k2 in range(nc):
-                            for w2 in range(nw):
-                                a = g0[:, :, k1, w1 + ext]  # a[x, y] = G0(x, y)
-                                b = g0[:, :, k2, w2 + ext]
-                                # L(b2,b1) = G0(b1,b2); L(b3,b4) = G0(b4,b3)
-                                blk = np.einsum("ij,lk->ijkl", a, b)  # (b1,b2,b3,b4) = G0(b1,b2) G0(b4,b3)
-                                out[:, :, :, :, k1, w1, k2, w2, iq, iv] += 2 * sign * beta ** 2 * blk
+                out[..., iq, iv] += 2 * sign * b2 * np.einsum("abkw,dclv->abcdkwlv", g1, g1)
     return out
 
 

@@ -1,0 +1,74 @@
+"""The C++ TpAccumulator mirror (include/g4acc.hpp) driven like the CT-AUX accumulator drives
+the reference class, against the FP64 oracle (GPU)."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from dca_amd import geometry as geo
+from dca_amd import synth
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(HERE, "cpp", "_build", "tp_accumulator_wrapper_test")
+
+
+def write_case(path, wl, measurements, expected):
+    g = wl.geometry
+    with open(path, "wb") as f:
+        np.array([g.nb, g.nc, g.n_freq, len(g.w_ex), len(g.k_ex), len(g.channels),
+                  len(measurements), g.g4_size()], np.int32).tofile(f)
+        np.array([g.beta], np.float64).tofile(f)
+        for a in (g.w_ex, g.k_ex, g.channels):
+            np.asarray(a, np.int32).tofile(f)
+        np.asarray(g.cluster.add.ravel(order="F"), np.int32).tofile(f)
+        np.asarray(g.cluster.sub.ravel(order="F"), np.int32).tofile(f)
+        np.asarray(g.cluster.dft.ravel(order="F"), np.complex128).tofile(f)
+        geo.g0_flat(g.g0).tofile(f)
+        for m in measurements:
+            np.array(m.n, np.int32).tofile(f)
+            np.array([m.sign], np.float64).tofile(f)
+            for s in range(2):
+                np.asarray(m.M[s], np.float64).ravel(order="F").tofile(f)
+                np.asarray(m.tau[s], np.float64).tofile(f)
+                np.asarray(m.site[s], np.int32).tofile(f)
+                np.asarray(m.band[s], np.int32).tofile(f)
+        for e in expected:
+            np.asarray(e, np.complex128).tofile(f)
+
+
+def test_case_file_layout(tmp_path):
+    """The case writer produces exactly what the C++ reader consumes (CPU check of the format)."""
+    wl = synth.tiny_workload(nb=1, L=2, n_freq=4, w_ex=(0,), k_ex=((0.0, 0.0),), n_range=(1, 3))
+    ms = wl.draw(2)
+    g = wl.geometry
+    exp = [np.zeros(g.g4_size(), np.complex128) for _ in g.channels]
+    p = tmp_path / "case.bin"
+    write_case(p, wl, ms, exp)
+    n_ext = g.n_ext
+    size = 8 * 4 + 8 + 4 * (len(g.w_ex) + len(g.k_ex) + len(g.channels) + 2 * g.nc ** 2) \
+        + 16 * (g.nc ** 2 + g.nb ** 2 * 2 * g.nc * n_ext) \
+        + sum(4 * 2 + 8 + sum(8 * n * n + 8 * n + 8 * n for n in m.n) for m in ms) \
+        + 16 * g.g4_size() * len(g.channels)
+    assert p.stat().st_size == size
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nb,L,precision,tol", [(1, 4, 0, 1e-12), (1, 4, 1, 1e-6), (3, 2, 0, 1e-12)])
+def test_cpp_mirror_matches_oracle(oracle_lib, tmp_path, nb, L, precision, tol):
+    assert os.path.exists(BIN), "build the driver with tests/cpp/build.sh (__graft_entry__.build())"
+    k_ex = ((0.0, 0.0), (math.pi, math.pi))
+    wl = synth.tiny_workload(nb=nb, L=L, n_freq=4, w_ex=(0, 1), k_ex=k_ex, n_range=(2, 16),
+                             seed_cfg=300 + nb)
+    ms = wl.draw(4)
+    ref = oracle_lib.OracleAccumulator(wl.geometry)
+    for m in ms:
+        ref.accumulate(m)
+    p = tmp_path / "case.bin"
+    write_case(p, wl, ms, ref.get_G4())
+    print(f"C++ mirror nb={nb} L={L} precision={precision}", flush=True)
+    r = subprocess.run([BIN, str(p), str(precision), str(tol)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0 and "OK" in r.stdout
