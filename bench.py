@@ -94,6 +94,18 @@ class DevicePool:
         return out
 
 
+def measured_traffic(cfg, walkers):
+    """HBM bytes per launch of the G4 kernel from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the same workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"g4_update_cfg{cfg}_hbm_traffic.txt")
+    if cfg != 2 or walkers != 64 or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def run_batches(acc, batches, steps):
     from dca_amd import g4acc
     lib, h = acc.lib, acc._h
@@ -204,6 +216,7 @@ def main():
     bytes_g4_stage = n_ch * 32 * g4_entries + W * 2 * g.D * g.D * 8
     achieved = bytes_g4_stage / (g4_ms * 1e-3) / 1e9
     stage = "g4" if g4_ms >= mtog_ms else "mtog"
+    traffic, traffic_src = measured_traffic(args.config, W)
 
     # ---- accuracy against the FP64 CPU oracle, and the CPU baseline ---------------------------
     result_err = {}
@@ -255,10 +268,11 @@ def main():
         "stage_ms_per_step": {"mixed_mtog": round(mtog_ms, 4), "mixed_g4": round(g4_ms, 4),
                               "plain_mtog": round(p_mtog, 4), "plain_g4": round(p_g4, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "G4 update stage (one launch per channel)",
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "G4 update (uv_kernel + g4_tiled_kernel, one launch each per batch)",
                      "dominant_stage": stage,
-                     "bytes_per_launch": bytes_g4_stage},
+                     "algorithmic_bytes_per_launch": bytes_g4_stage,
+                     "traffic_source": traffic_src},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
