@@ -14,8 +14,9 @@
 //   CHARGE      += su sv - a (X_0 Y_0 + X_1 Y_1)
 //   PP_UP_DOWN  += a (X_0 Z_1 + X_1 Z_0)
 //   TRANSVERSE  -= a (X_0 Y_1 + X_1 Y_0)
-// Products are formed in the G precision; each walker's term is added to FP64 accumulators;
-// G4 is read and written once per launch (FP64, or FP32 value + Kahan compensation).
+// Products are formed in the G precision and summed per term type over 8 walkers before they are
+// added to FP64 accumulators; G4 is read and written once per launch (FP64, or FP32 value + Kahan
+// compensation).
 #include "g4acc_common.hpp"
 #include "g4acc_kernels.hpp"
 
@@ -156,19 +157,42 @@ __global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> 
                        static_cast<size_t>(nc) * (u_plane < 2 ? w1 : w2);
   const bool u_valid = u_ok && (q0 + u_qi < g.nq);
 
-  double2 acc[PPT][QG][NCH];
+  // Channel outputs are FP64 totals. With complex64 G the walker terms are first summed in FP32
+  // per term type for FLUSH walkers (relative rounding ~2^-24 per step), so the per-walker work
+  // is fused multiply-adds only:
+  //   xs = a(X0 Y0 + X1 Y1), pz = a(X0 Z1 + X1 Z0), tr = a(X0 Y1 + X1 Y0), dm = du dv, dc = su sv
+  //   MAGNETIC = dm - xs, CHARGE = dc - xs, PP = pz, TRANSVERSE = -tr,
+  // and the FP64 additions happen once per FLUSH walkers. With complex128 G every walker's
+  // channel term goes straight into the totals.
+  constexpr bool kChunk = sizeof(Real) == 4;
+  constexpr int FLUSH = 8;
+  constexpr int PC = kChunk ? PPT : 1, QC = kChunk ? QG : 1;
+  constexpr bool kPrefetchX = PPT <= 4;  // registers: many pairs per thread read X in place
+  const bool has_trans = a.has_trans;
+  bool has_mag = false, has_chg = false;
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    has_mag |= a.ch[c] == PH_MAGNETIC;
+    has_chg |= a.ch[c] == PH_CHARGE;
+  }
+  double2 tot[PPT][QG][NCH];
+  C xs[PC][QC], pz[PC][QC], tr[PC][QC], dm[PC][QC], dc[PC][QC];
 #pragma unroll
   for (int j = 0; j < PPT; ++j)
 #pragma unroll
     for (int qi = 0; qi < QG; ++qi)
 #pragma unroll
       for (int c = 0; c < NCH; ++c)
-        acc[j][qi][c] = double2{0, 0};
-
+        tot[j][qi][c] = double2{0, 0};
+#pragma unroll
+  for (int j = 0; j < PC; ++j)
+#pragma unroll
+    for (int qi = 0; qi < QC; ++qi)
+      xs[j][qi] = pz[j][qi] = tr[j][qi] = dm[j][qi] = dc[j][qi] = C{0, 0};
   // Everything walker m needs is loaded into registers while walker m - 1 is consumed: the
   // staged blocks (each thread stages at most PPT elements per block, nc^2 <= 256 PPT), the
-  // rank-one factors and the thread's own X entries.
-  C pre[4][PPT], pre_uv, pre_x[2][PPT];
+  // rank-one factors and (for few pairs per thread) the thread's own X entries.
+  C pre[4][PPT], pre_uv, pre_x[2][kPrefetchX ? PPT : 1];
   auto prefetch = [&](int m) {
     const C* H0 = a.G + (2 * static_cast<size_t>(m)) * DD;
     const C* H1 = H0 + DD;
@@ -186,10 +210,12 @@ __global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> 
           pre[3][j] = H1[zoff + go];
         }
       }
-      if (ok[j]) {
-        const size_t xo = xoff + k1[j] + D * k2[j];
-        pre_x[0][j] = H0[xo];
-        pre_x[1][j] = H1[xo];
+      if constexpr (kPrefetchX) {
+        if (ok[j]) {
+          const size_t xo = xoff + k1[j] + D * k2[j];
+          pre_x[0][j] = H0[xo];
+          pre_x[1][j] = H1[xo];
+        }
       }
     }
     if (u_valid)
@@ -216,21 +242,33 @@ __global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> 
     }
     if (u_valid)
       Ul[threadIdx.x] = pre_uv;
-    C xs[2][PPT];
+    C xw[2][kPrefetchX ? PPT : 1];
+    if constexpr (kPrefetchX) {
 #pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-      xs[0][j] = pre_x[0][j];
-      xs[1][j] = pre_x[1][j];
+      for (int j = 0; j < PPT; ++j) {
+        xw[0][j] = pre_x[0][j];
+        xw[1][j] = pre_x[1][j];
+      }
     }
     __syncthreads();
     const Real alpha = static_cast<Real>(0.5 * a.desc[2 * m].sign);
+    const C* G0m = a.G + (2 * static_cast<size_t>(m)) * DD;
     if (m + 1 < a.n_walkers)
       prefetch(m + 1);
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
       if (!ok[j])
         continue;
-      C x0 = xs[0][j], x1 = xs[1][j];
+      C x0, x1;
+      if constexpr (kPrefetchX) {
+        x0 = xw[0][j];
+        x1 = xw[1][j];
+      }
+      else {
+        const size_t xo = xoff + k1[j] + D * k2[j];
+        x0 = G0m[xo];
+        x1 = G0m[DD + xo];
+      }
       x0.x *= alpha;
       x0.y *= alpha;
       x1.x *= alpha;
@@ -239,33 +277,89 @@ __global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> 
       for (int qi = 0; qi < QG; ++qi) {
         if (!qok[qi])
           continue;
-        C cross{0, 0}, ppz{0, 0}, trans{0, 0};
-        if (needY) {
-          const C y0 = Yl[yi[j][qi]], y1 = Yl[bs + yi[j][qi]];
-          cross = cadd(cmul(x0, y0), cmul(x1, y1));
-          if (a.has_trans)
-            trans = cadd(cmul(x0, y1), cmul(x1, y0));
-        }
-        if (needZ) {
-          const C z0 = Zl[zi[j][qi]], z1 = Zl[bs + zi[j][qi]];
-          ppz = cadd(cmul(x0, z1), cmul(x1, z0));
-        }
         const C* uq = Ul + qi * nc;
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-          C t;
-          const int ch = a.ch[c];
-          if (ch == PH_MAGNETIC)
-            t = csub(cmul(uq[k1[j]], uq[2 * QG * nc + k2[j]]), cross);
-          else if (ch == PH_CHARGE)
-            t = csub(cmul(uq[QG * nc + k1[j]], uq[3 * QG * nc + k2[j]]), cross);
-          else if (ch == PP_UP_DOWN)
-            t = ppz;
-          else
-            t = C{-trans.x, -trans.y};
-          acc[j][qi][c].x += static_cast<double>(t.x);
-          acc[j][qi][c].y += static_cast<double>(t.y);
+        if constexpr (kChunk) {
+          const int jj = kChunk ? j : 0, qq = kChunk ? qi : 0;
+          if (needY) {
+            const C y0 = Yl[yi[j][qi]], y1 = Yl[bs + yi[j][qi]];
+            cfma(xs[jj][qq], x0, y0);
+            cfma(xs[jj][qq], x1, y1);
+            if (has_trans) {
+              cfma(tr[jj][qq], x0, y1);
+              cfma(tr[jj][qq], x1, y0);
+            }
+          }
+          if (needZ) {
+            const C z0 = Zl[zi[j][qi]], z1 = Zl[bs + zi[j][qi]];
+            cfma(pz[jj][qq], x0, z1);
+            cfma(pz[jj][qq], x1, z0);
+          }
+          if (has_mag)
+            cfma(dm[jj][qq], uq[k1[j]], uq[2 * QG * nc + k2[j]]);
+          if (has_chg)
+            cfma(dc[jj][qq], uq[QG * nc + k1[j]], uq[3 * QG * nc + k2[j]]);
         }
+        else {
+          C cross{0, 0}, ppz{0, 0}, trans{0, 0};
+          if (needY) {
+            const C y0 = Yl[yi[j][qi]], y1 = Yl[bs + yi[j][qi]];
+            cross = cmul(x0, y0);
+            cfma(cross, x1, y1);
+            if (has_trans) {
+              trans = cmul(x0, y1);
+              cfma(trans, x1, y0);
+            }
+          }
+          if (needZ) {
+            const C z0 = Zl[zi[j][qi]], z1 = Zl[bs + zi[j][qi]];
+            ppz = cmul(x0, z1);
+            cfma(ppz, x1, z0);
+          }
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) {
+            const int ch = a.ch[c];
+            C t;
+            if (ch == PH_MAGNETIC) {
+              t = csub(C{0, 0}, cross);
+              cfma(t, uq[k1[j]], uq[2 * QG * nc + k2[j]]);
+            }
+            else if (ch == PH_CHARGE) {
+              t = csub(C{0, 0}, cross);
+              cfma(t, uq[QG * nc + k1[j]], uq[3 * QG * nc + k2[j]]);
+            }
+            else if (ch == PP_UP_DOWN)
+              t = ppz;
+            else
+              t = C{-trans.x, -trans.y};
+            tot[j][qi][c].x += static_cast<double>(t.x);
+            tot[j][qi][c].y += static_cast<double>(t.y);
+          }
+        }
+      }
+    }
+    if constexpr (kChunk) {
+      if ((m + 1) % FLUSH == 0 || m + 1 == a.n_walkers) {
+#pragma unroll
+        for (int j = 0; j < PC; ++j)
+#pragma unroll
+          for (int qi = 0; qi < QC; ++qi) {
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+              const int ch = a.ch[c];
+              C t;
+              if (ch == PH_MAGNETIC)
+                t = csub(dm[j][qi], xs[j][qi]);
+              else if (ch == PH_CHARGE)
+                t = csub(dc[j][qi], xs[j][qi]);
+              else if (ch == PP_UP_DOWN)
+                t = pz[j][qi];
+              else
+                t = C{-tr[j][qi].x, -tr[j][qi].y};
+              tot[j][qi][c].x += static_cast<double>(t.x);
+              tot[j][qi][c].y += static_cast<double>(t.y);
+            }
+            xs[j][qi] = pz[j][qi] = tr[j][qi] = dm[j][qi] = dc[j][qi] = C{0, 0};
+          }
       }
     }
   }
@@ -283,7 +377,7 @@ __global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> 
                                                             (w2 + static_cast<size_t>(nw) * qv[qi])));
 #pragma unroll
       for (int c = 0; c < NCH; ++c)
-        g4_store_t(a.g4[c], idx, acc[j][qi][c], a.sum_mode);
+        g4_store_t(a.g4[c], idx, tot[j][qi][c], a.sum_mode);
     }
   }
 }

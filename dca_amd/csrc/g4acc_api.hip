@@ -11,6 +11,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "g4acc_common.hpp"
@@ -62,6 +63,9 @@ struct g4acc_handle {
   size_t stage_f_cap = 0, stage_i_cap = 0;
   hipEvent_t staged = nullptr;     // staging buffers free once this event completes
   bool staged_pending = false;
+  // Captured launch sequences, one per batch size (G4ACC_GRAPH=0 disables them).
+  bool use_graph = true;
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;
   // stage timing (profiling hook)
   bool timing = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -309,8 +313,50 @@ void run_g4(g4acc_handle* h, int n_walkers) {
   }
 }
 
+void launch_sequence(g4acc_handle* h, int n_walkers) {
+  if (h->precision == G4ACC_FP64) {
+    run_mtog<double>(h, 2 * n_walkers);
+    run_g4<double>(h, n_walkers);
+  }
+  else {
+    run_mtog<float>(h, 2 * n_walkers);
+    run_g4<float>(h, n_walkers);
+  }
+}
+
+// The kernels of one batch read everything batch-specific (vertex counts, pointers, signs) from
+// the slot descriptors, so the sequence for a given batch size is captured once into a hipGraph
+// and replayed; only the descriptor copy precedes each replay. The sequence holds kernel launches
+// only (no memset nodes).
+int run_batch_graph(g4acc_handle* h, int n_walkers) {
+  hipGraphExec_t exec = nullptr;
+  for (auto& e : h->graphs)
+    if (e.first == n_walkers)
+      exec = e.second;
+  if (!exec) {
+    hipGraph_t graph = nullptr;
+    G4_TRY(h->check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal),
+                    "hipStreamBeginCapture"));
+    launch_sequence(h, n_walkers);
+    const hipError_t launched = hipGetLastError();
+    const hipError_t ended = hipStreamEndCapture(h->stream, &graph);
+    if (launched != hipSuccess || ended != hipSuccess || !graph) {
+      if (graph)
+        (void)hipGraphDestroy(graph);
+      return h->check(launched != hipSuccess ? launched : ended, "graph capture");
+    }
+    const hipError_t inst = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    G4_TRY(h->check(inst, "hipGraphInstantiate"));
+    h->graphs.emplace_back(n_walkers, exec);
+  }
+  return h->check(hipGraphLaunch(exec, h->stream), "hipGraphLaunch");
+}
+
 int run_batch(g4acc_handle* h, int n_walkers) {
   const int n_slots = 2 * n_walkers;
+  if (h->use_graph && !h->timing)
+    return run_batch_graph(h, n_walkers);
   if (h->timing)
     G4_TRY(h->check(hipEventRecord(h->ev[0], h->stream), "hipEventRecord"));
   if (h->precision == G4ACC_FP64)
@@ -466,6 +512,8 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
     const size_t lds = tiled_lds_bytes(g, need_y, need_z, cb) + cb * 256;
     if (const char* to = getenv("G4ACC_TILE_ORDER"))
       h->tile_order = atoi(to);
+    if (const char* gr = getenv("G4ACC_GRAPH"))
+      h->use_graph = atoi(gr) != 0;
     h->use_tiled = g.nb == 1 && g.nc >= 8 && tiled_pairs_per_thread(g.nc) > 0 &&
                    lds <= static_cast<size_t>(max_lds) && getenv("G4ACC_GENERIC_G4") == nullptr;
     if (h->use_tiled &&
@@ -524,6 +572,8 @@ int g4acc_destroy(g4acc_handle* h) {
     return G4ACC_OK;
   if (h->stream)
     (void)hipStreamSynchronize(h->stream);
+  for (auto& e : h->graphs)
+    (void)hipGraphExecDestroy(e.second);
   for (void* p : h->allocations)
     (void)hipFree(p);
   if (h->h_desc)

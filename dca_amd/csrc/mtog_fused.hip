@@ -59,6 +59,9 @@ __global__ void __launch_bounds__(256) mtog_left_fused(const SlotDesc* __restric
   }
 }
 
+// The phases of the workgroup's 4 TW frequencies (for a chunk of JC sorted vertices), the
+// cluster DFT matrix and the G0 slice of those frequencies are staged in LDS and read as
+// broadcasts; A rows are read from global memory, four in flight per step.
 template <typename Real, int NC, int TW>
 __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restrict__ desc, Geo g,
                                                         int n_max, int slot0,
@@ -70,20 +73,30 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
                                                         const double2* __restrict__ G0,
                                                         Cplx<Real>* __restrict__ G) {
   using C = Cplx<Real>;
+  constexpr int WT = 4 * TW;     // frequencies per workgroup
+  constexpr int JC = 4096 / WT;  // vertices per staged phase chunk (4096 staged phases)
+  __shared__ C Tl[JC * WT];
+  __shared__ C Dl[NC * NC];
+  __shared__ double2 G0l[NC * WT];
   const int slot = slot0 + blockIdx.z;
   const int s = slot & 1;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int p1 = blockIdx.x * 64 + lane;
-  const int w2b = (blockIdx.y * 4 + wave) * TW;
-  if (w2b >= g.nwE)
-    return;
-  if (p1 >= g.D)
-    return;
-  const int pr = p1;
+  const int w2wg = blockIdx.y * WT;           // first frequency of the workgroup
+  const int w2b = w2wg + wave * TW;           // first frequency of this wave
+  const int n = desc[slot].n;
+  const bool active = p1 < g.D && w2b < g.nwE;
   const int* O = off + static_cast<size_t>(slot) * (NC + 1);
   const C* Ts = T + static_cast<size_t>(slot) * g.nwE * n_max;
-  const C* As = A + static_cast<size_t>(blockIdx.z) * g.D * n_max + pr;
+  const C* As = A + static_cast<size_t>(blockIdx.z) * g.D * n_max + (p1 < g.D ? p1 : 0);
+
+  for (int i = threadIdx.x; i < NC * NC; i += 256)
+    Dl[i] = dft[i];
+  for (int i = threadIdx.x; i < NC * WT; i += 256) {
+    const int k = i % NC, wl = i / NC, w = w2wg + wl;
+    G0l[i] = w < g.nwE ? G0[s + 2 * (k + static_cast<size_t>(NC) * w)] : double2{0, 0};
+  }
 
   C e[TW][NC];
 #pragma unroll
@@ -91,39 +104,53 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
 #pragma unroll
     for (int r = 0; r < NC; ++r)
       e[tt][r] = C{0, 0};
-  const int tw_ok = g.nwE - w2b;  // frequencies of this group inside the domain
+  const int tw_ok = g.nwE - w2b;  // frequencies of this wave inside the domain
+  for (int c0 = 0; c0 < n; c0 += JC) {
+    const int c1 = c0 + JC < n ? c0 + JC : n;
+    __syncthreads();
+    for (int i = threadIdx.x; i < (c1 - c0) * WT; i += 256) {
+      const int wl = i % WT, jl = i / WT, w = w2wg + wl;
+      Tl[i] = w < g.nwE ? Ts[w + static_cast<size_t>(g.nwE) * (c0 + jl)] : C{0, 0};
+    }
+    __syncthreads();
+    if (!active)
+      continue;
 #pragma unroll
-  for (int r = 0; r < NC; ++r) {
-    const int j0 = O[r], j1 = O[r + 1];
-    int jj = j0;
-    // Four independent row loads in flight per step.
-    for (; jj + 4 <= j1; jj += 4) {
-      C av[4];
+    for (int r = 0; r < NC; ++r) {
+      const int j0 = O[r] > c0 ? O[r] : c0;
+      const int j1 = O[r + 1] < c1 ? O[r + 1] : c1;
+      int jj = j0;
+      for (; jj + 4 <= j1; jj += 4) {
+        C av[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        av[u] = As[static_cast<size_t>(g.D) * (jj + u)];
+        for (int u = 0; u < 4; ++u)
+          av[u] = As[static_cast<size_t>(g.D) * (jj + u)];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const C* tcol = Ts + static_cast<size_t>(g.nwE) * (jj + u) + w2b;
+        for (int u = 0; u < 4; ++u) {
+          const C* tl = Tl + (jj + u - c0) * WT + wave * TW;
+#pragma unroll
+          for (int tt = 0; tt < TW; ++tt)
+            if (tt < tw_ok)
+              cfmac(e[tt][r], av[u], tl[tt]);
+        }
+      }
+      for (; jj < j1; ++jj) {
+        const C av = As[static_cast<size_t>(g.D) * jj];
+        const C* tl = Tl + (jj - c0) * WT + wave * TW;
 #pragma unroll
         for (int tt = 0; tt < TW; ++tt)
           if (tt < tw_ok)
-            cfmac(e[tt][r], av[u], tcol[tt]);
+            cfmac(e[tt][r], av, tl[tt]);
       }
     }
-    for (; jj < j1; ++jj) {
-      const C a = As[static_cast<size_t>(g.D) * jj];
-      const C* tcol = Ts + static_cast<size_t>(g.nwE) * jj + w2b;
-#pragma unroll
-      for (int tt = 0; tt < TW; ++tt)
-        if (tt < tw_ok)
-          cfmac(e[tt][r], a, tcol[tt]);
-    }
   }
+  if (n == 0)
+    __syncthreads();  // Dl / G0l visible (no chunk loop ran)
+  if (!active)
+    return;
   // Epilogue: right site DFT, band phases, G = -G0 M~ G0 + beta G0 delta (FP64).
   const int k1 = p1 % NC, w1 = p1 / NC;
-  auto g0 = [&](int k, int w) { return G0[s + 2 * (k + static_cast<size_t>(NC) * w)]; };
-  const double2 ga = g0(k1, w1);
+  const double2 ga = G0[s + 2 * (k1 + static_cast<size_t>(NC) * w1)];
   const double inv_nc = 1.0 / NC;
   C* Gs = G + static_cast<size_t>(slot) * g.D * g.D + p1;
 #pragma unroll
@@ -136,13 +163,13 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
       C acc{0, 0};
 #pragma unroll
       for (int r = 0; r < NC; ++r)
-        cfmac(acc, e[tt][r], dft[k2 + NC * r]);
+        cfmac(acc, e[tt][r], Dl[k2 + NC * r]);
       double2 m = to_d(acc);
       m.x *= inv_nc;
       m.y *= inv_nc;
       if (bphase)
         m = cmulc(cmul(m, bphase[k1]), bphase[k2]);
-      double2 v = cmul(cmul(ga, m), g0(k2, w2));
+      double2 v = cmul(cmul(ga, m), G0l[k2 + NC * (w2 - w2wg)]);
       v.x = -v.x;
       v.y = -v.y;
       const int p2 = k2 + NC * w2;

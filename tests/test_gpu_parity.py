@@ -208,7 +208,9 @@ def test_tiled_equals_generic_kernel(name, precision, monkeypatch):
     for a, b in zip(tiled.get_G4(), generic.get_G4()):
         err = linf(b, a)
         print(f"{name} {precision} tiled vs generic l_inf={err:.3e}", flush=True)
-        assert err < (1e-14 if precision == "fp64" else 1e-7)
+        # FP32: the tiled kernel sums 8 walker terms in FP32 before its FP64 flush, the generic
+        # kernel adds every term in FP64.
+        assert err < (1e-14 if precision == "fp64" else 5e-7)
 
 
 @pytest.mark.parametrize("name", ["1band_2x2", "1band_4x4_4q", "1band_6x6", "1band_8site"])
@@ -224,3 +226,29 @@ def test_fused_equals_unfused_mtog(name, precision, monkeypatch):
         err = linf(b, a)
         print(f"{name} {precision} fused vs unfused G l_inf={err:.3e}", flush=True)
         assert err < (1e-14 if precision == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("name", ["1band_4x4_4q", "3band_2x2"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_graph_replays_equal_eager(name, precision, monkeypatch):
+    """Batches replayed from the captured hipGraph give bit-identical G4 to eager launches, on
+    every replay, with host synchronisation and eager work between the replays."""
+    wl = case(name)
+    ms = wl.draw(12)
+    graph = g4acc.TpAccumulator(wl.geometry, precision=precision, max_walkers=4, max_vertices=wl.n_max)
+    monkeypatch.setenv("G4ACC_GRAPH", "0")
+    eager = g4acc.TpAccumulator(wl.geometry, precision=precision, max_walkers=4, max_vertices=wl.n_max)
+    graph.resetAccumulation()
+    eager.resetAccumulation()
+    for i in range(3):
+        batch = ms[4 * i:4 * i + 4]
+        graph.accumulate_batch(batch)
+        eager.accumulate_batch(batch)
+        eager.compute_G(batch[0])  # unrelated eager work on the graph's buffers between replays
+        graph.compute_G(batch[1])
+        for a, b in zip(graph.get_G4(), eager.get_G4()):
+            assert np.array_equal(a, b), f"replay {i}"
+    graph.accumulate_batch(ms[:3])  # another batch size: a second captured sequence
+    eager.accumulate_batch(ms[:3])
+    for a, b in zip(graph.get_G4(), eager.get_G4()):
+        assert np.array_equal(a, b)
