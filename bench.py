@@ -49,51 +49,6 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-class DevicePool:
-    """All measurements of the pool resident in HBM (torch is only the allocator here)."""
-
-    def __init__(self, torch, measurements):
-        from dca_amd import g4acc
-        self.measurements = measurements
-        Ms, taus, sites, bands = [], [], [], []
-        offs = []
-        om = ov = 0
-        for meas in measurements:
-            o = []
-            for s in range(2):
-                n = meas.n[s]
-                o.append((om, ov, n))
-                Ms.append(np.asarray(meas.M[s], np.float64).ravel(order="F"))
-                taus.append(meas.tau[s])
-                sites.append(meas.site[s])
-                bands.append(meas.band[s])
-                om += n * n
-                ov += n
-            offs.append(o)
-        dev = torch.device("cuda:0")
-        self.M = torch.from_numpy(np.concatenate(Ms)).to(dev)
-        self.tau = torch.from_numpy(np.concatenate(taus).astype(np.float64)).to(dev)
-        self.site = torch.from_numpy(np.concatenate(sites).astype(np.int32)).to(dev)
-        self.band = torch.from_numpy(np.concatenate(bands).astype(np.int32)).to(dev)
-        torch.cuda.synchronize()
-        pm, pt, ps, pb = (self.M.data_ptr(), self.tau.data_ptr(), self.site.data_ptr(),
-                          self.band.data_ptr())
-        self.walkers = []
-        for meas, o in zip(measurements, offs):
-            self.walkers.append(g4acc.device_walker_struct(
-                meas.n, [pm + 8 * o[s][0] for s in range(2)], [max(1, o[s][2]) for s in range(2)],
-                [pt + 8 * o[s][1] for s in range(2)], [ps + 4 * o[s][1] for s in range(2)],
-                [pb + 4 * o[s][1] for s in range(2)], meas.sign))
-
-    def batches(self, size):
-        from dca_amd import g4acc
-        out = []
-        for i in range(0, len(self.walkers) - size + 1, size):
-            arr = (g4acc.Walker_c * size)(*self.walkers[i:i + size])
-            out.append((arr, size))
-        return out
-
-
 def measured_traffic(cfg, walkers):
     """HBM bytes per launch of the G4 kernel from the committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the same workload, or None."""
@@ -163,6 +118,7 @@ def main():
     t0 = time.perf_counter()
     pool = wl.draw(W * args.pool)
     log(f"drew {len(pool)} measurements in {time.perf_counter() - t0:.1f}s")
+    from dca_amd.devpool import DevicePool
     dp = DevicePool(torch, pool)
     batches = dp.batches(W)
 

@@ -17,6 +17,8 @@
 // Products are formed in the G precision and summed per term type over 8 walkers before they are
 // added to FP64 accumulators; G4 is read and written once per launch (FP64, or FP32 value + Kahan
 // compensation).
+#include <type_traits>
+
 #include "g4acc_common.hpp"
 #include "g4acc_kernels.hpp"
 
@@ -24,7 +26,6 @@ namespace g4acc {
 
 namespace {
 constexpr int PH_MAGNETIC = 2, PH_CHARGE = 3, PP_UP_DOWN = 4;  // else: PH_TRANSVERSE (1)
-constexpr int TILE_THREADS = 256;
 
 __device__ __forceinline__ void g4_store_t(void* g4, size_t i, double2 acc, int sum_mode) {
   if (sum_mode == SUM_FP64) {
@@ -80,8 +81,9 @@ __global__ void __launch_bounds__(256) uv_kernel(TiledArgs<Real> a) {
   out[3 * plane] = C{v0.x + v1.x, v0.y + v1.y};
 }
 
-template <typename Real, int PPT, int QG, int NCH>
-__global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> a) {
+// MASK: compile-time channel set (bit 1 << type), or 0 to read the channel flags at run time.
+template <typename Real, int TT, int PPT, int QG, int NCH, int MASK>
+__global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
   using C = Cplx<Real>;
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
@@ -115,7 +117,10 @@ __global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> 
   const int iv = z / a.nqg, q0 = (z % a.nqg) * QG;
   const int nu = a.w_ex[iv];
   const int ld = nc + 1, bs = nc * ld;
-  const bool needY = a.need_y, needZ = a.need_z;
+  constexpr bool kStatic = MASK != 0;
+  constexpr int kY = (1 << PH_MAGNETIC) | (1 << PH_CHARGE) | (1 << 1);
+  const bool needY = kStatic ? (MASK & kY) != 0 : a.need_y;
+  const bool needZ = kStatic ? (MASK & (1 << PP_UP_DOWN)) != 0 : a.need_z;
   C* Yl = lds;                        // [spin][b][a], a fastest
   C* Zl = Yl + (needY ? 2 * bs : 0);
   C* Ul = Zl + (needZ ? 2 * bs : 0);  // [plane][qi][k]: a(u0-u1), a(u0+u1), v0-v1, v0+v1
@@ -137,7 +142,7 @@ __global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> 
   }
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
-    const int pi = threadIdx.x + TILE_THREADS * j;
+    const int pi = threadIdx.x + TT * j;
     ok[j] = pi < nc * nc;
     k1[j] = ok[j] ? pi % nc : 0;
     k2[j] = ok[j] ? pi / nc : 0;
@@ -167,13 +172,18 @@ __global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> 
   constexpr bool kChunk = sizeof(Real) == 4;
   constexpr int FLUSH = 8;
   constexpr int PC = kChunk ? PPT : 1, QC = kChunk ? QG : 1;
-  constexpr bool kPrefetchX = PPT <= 4;  // registers: many pairs per thread read X in place
-  const bool has_trans = a.has_trans;
-  bool has_mag = false, has_chg = false;
+  // Registers: with many pairs per thread (or 1024-thread workgroups, 128 VGPRs) X is read in place.
+  constexpr bool kPrefetchX = PPT * (TT / 256) <= 4 && !(TT > 256 && PPT > 1);
+  bool has_trans = a.has_trans, has_mag = false, has_chg = false;
 #pragma unroll
   for (int c = 0; c < NCH; ++c) {
     has_mag |= a.ch[c] == PH_MAGNETIC;
     has_chg |= a.ch[c] == PH_CHARGE;
+  }
+  if constexpr (kStatic) {
+    has_trans = (MASK & (1 << 1)) != 0;
+    has_mag = (MASK & (1 << PH_MAGNETIC)) != 0;
+    has_chg = (MASK & (1 << PH_CHARGE)) != 0;
   }
   double2 tot[PPT][QG][NCH];
   C xs[PC][QC], pz[PC][QC], tr[PC][QC], dm[PC][QC], dc[PC][QC];
@@ -198,7 +208,7 @@ __global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> 
     const C* H1 = H0 + DD;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-      const int idx = threadIdx.x + TILE_THREADS * j;
+      const int idx = threadIdx.x + TT * j;
       if (idx < nc * nc) {
         const size_t go = (idx % nc) + D * (idx / nc);
         if (needY) {
@@ -227,7 +237,7 @@ __global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> 
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-      const int idx = threadIdx.x + TILE_THREADS * j;
+      const int idx = threadIdx.x + TT * j;
       if (idx < nc * nc) {
         const int li = (idx / nc) * ld + (idx % nc);
         if (needY) {
@@ -384,7 +394,7 @@ __global__ void __launch_bounds__(TILE_THREADS) g4_tiled_kernel(TiledArgs<Real> 
 
 namespace {
 
-template <typename Real, int PPT, int QG, int NCH>
+template <typename Real, int TT, int PPT, int QG, int NCH, int MASK>
 void launch_tiled_fixed(TiledArgs<Real> a, size_t lds, hipStream_t st) {
   const Geo& g = a.g;
   unsigned tiles;
@@ -399,29 +409,56 @@ void launch_tiled_fixed(TiledArgs<Real> a, size_t lds, hipStream_t st) {
     a.n_xcd = 0;
     tiles = static_cast<unsigned>(g.nw) * g.nw * g.nv * a.nqg;
   }
-  hipLaunchKernelGGL((g4_tiled_kernel<Real, PPT, QG, NCH>), dim3(tiles), dim3(TILE_THREADS), lds, st, a);
+  hipLaunchKernelGGL((g4_tiled_kernel<Real, TT, PPT, QG, NCH, MASK>), dim3(tiles), dim3(TT), lds, st, a);
 }
 
-template <typename Real, int PPT, int QG>
+// Channel sets of the measured workloads get their own instantiations (registers sized for the
+// terms they need); any other set reads the channel flags at run time.
+template <typename Real, int TT, int PPT, int QG>
 bool launch_tiled_nch(const TiledArgs<Real>& a, size_t lds, hipStream_t st) {
+  int mask = 0;
+  for (int c = 0; c < a.n_ch; ++c)
+    mask |= 1 << a.ch[c];
+  constexpr int M = 1 << PH_MAGNETIC, Q = 1 << PH_CHARGE, P = 1 << PP_UP_DOWN;
+  switch (mask) {
+    case M | Q | P: launch_tiled_fixed<Real, TT, PPT, QG, 3, M | Q | P>(a, lds, st); return true;
+    case M | P: launch_tiled_fixed<Real, TT, PPT, QG, 2, M | P>(a, lds, st); return true;
+    case Q: launch_tiled_fixed<Real, TT, PPT, QG, 1, Q>(a, lds, st); return true;
+    case M: launch_tiled_fixed<Real, TT, PPT, QG, 1, M>(a, lds, st); return true;
+    default: break;
+  }
   switch (a.n_ch) {
-    case 1: launch_tiled_fixed<Real, PPT, QG, 1>(a, lds, st); return true;
-    case 2: launch_tiled_fixed<Real, PPT, QG, 2>(a, lds, st); return true;
-    case 3: launch_tiled_fixed<Real, PPT, QG, 3>(a, lds, st); return true;
-    case 4: launch_tiled_fixed<Real, PPT, QG, 4>(a, lds, st); return true;
+    case 1: launch_tiled_fixed<Real, TT, PPT, QG, 1, 0>(a, lds, st); return true;
+    case 2: launch_tiled_fixed<Real, TT, PPT, QG, 2, 0>(a, lds, st); return true;
+    case 3: launch_tiled_fixed<Real, TT, PPT, QG, 3, 0>(a, lds, st); return true;
+    case 4: launch_tiled_fixed<Real, TT, PPT, QG, 4, 0>(a, lds, st); return true;
     default: return false;
   }
 }
 
+// Workgroup shape: 256 threads while a thread owns at most 2 of the nc^2 (k1, k2) pairs,
+// 1024 threads (at most 4 pairs each) for larger clusters.
+struct TileShape {
+  int threads, ppt;
+};
+TileShape tile_shape(int nc) {
+  const int pairs = nc * nc;
+  if (pairs <= 256)
+    return {256, 1};
+  if (pairs <= 512)
+    return {256, 2};
+  if (pairs <= 1024)
+    return {1024, 1};
+  if (pairs <= 2048)
+    return {1024, 2};
+  if (pairs <= 4096)
+    return {1024, 4};
+  return {0, 0};
+}
+
 }  // namespace
 
-int tiled_pairs_per_thread(int nc) {
-  const int pairs = nc * nc;
-  for (int p : {1, 2, 4, 6, 8, 16})
-    if (p * TILE_THREADS >= pairs)
-      return p;
-  return 0;
-}
+int tiled_pairs_per_thread(int nc) { return tile_shape(nc).ppt; }
 
 size_t tiled_lds_bytes(const Geo& g, bool need_y, bool need_z, size_t cplx_bytes) {
   const size_t bs = static_cast<size_t>(g.nc) * (g.nc + 1);
@@ -431,40 +468,49 @@ size_t tiled_lds_bytes(const Geo& g, bool need_y, bool need_z, size_t cplx_bytes
 template <typename Real>
 bool launch_g4_tiled(TiledArgs<Real> a, hipStream_t st) {
   const Geo& g = a.g;
-  const int ppt = tiled_pairs_per_thread(g.nc);
+  const TileShape ts = tile_shape(g.nc);
+  const int ppt = ts.ppt;
   if (g.nb != 1 || ppt == 0 || a.n_ch < 1 || a.n_ch > 4)
     return false;
-  // Momenta per workgroup: as many as fit 16 FP64 accumulators per thread.
-  int qg = 1;
-  if (ppt == 1)
-    qg = g.nq >= 4 ? 4 : (g.nq >= 2 ? 2 : 1);
-  else if (ppt == 2)
-    qg = g.nq >= 2 ? 2 : 1;
-  while (qg > 1 && ppt * qg * a.n_ch > 16)
+  // Momenta per workgroup: as many as fit 16 FP64 accumulators per thread (and as the
+  // instantiated shapes provide: 4 for one pair per thread, 2 for two, else 1).
+  int qg = g.nq >= 4 ? 4 : (g.nq >= 2 ? 2 : 1);
+  if (ppt == 2 && qg > 2)
+    qg = 2;
+  if (ppt > 2)
+    qg = 1;
+  // FP64 accumulators per thread: 16 for 256-thread workgroups, 4 for 1024-thread ones.
+  const int acc_budget = ts.threads == 256 ? 16 : 4;
+  while (qg > 1 && ppt * qg * a.n_ch > acc_budget)
     qg /= 2;
-  if (ppt * qg * a.n_ch > 24 || 4 * qg * g.nc > TILE_THREADS)
+  while (a.qg_cap > 0 && qg > a.qg_cap)
+    qg /= 2;
+  if (ppt * qg * a.n_ch > 16 || 4 * qg * g.nc > ts.threads)
     return false;  // accumulators would not fit in registers / one staged factor per thread
   a.nqg = (g.nq + qg - 1) / qg;
   const size_t lds = tiled_lds_bytes(g, a.need_y, a.need_z, sizeof(Cplx<Real>)) +
                      sizeof(Cplx<Real>) * 4 * qg * g.nc;
   const dim3 ug((g.nc * g.nw + 255) / 256, g.nq * g.nv, a.n_walkers);
   hipLaunchKernelGGL(uv_kernel<Real>, ug, dim3(256), 0, st, a);
-#define G4_TILED_CASE(PPT_)                                                   \
-  case PPT_:                                                                  \
-    if (qg == 4) return launch_tiled_nch<Real, PPT_, (PPT_ <= 1 ? 4 : 1)>(a, lds, st); \
-    if (qg == 2) return launch_tiled_nch<Real, PPT_, (PPT_ <= 2 ? 2 : 1)>(a, lds, st); \
-    return launch_tiled_nch<Real, PPT_, 1>(a, lds, st);
+  auto by_qg = [&](auto tt, auto pp) {
+    constexpr int TT = decltype(tt)::value, PP = decltype(pp)::value;
+    if (qg == 4)
+      return launch_tiled_nch<Real, TT, PP, (PP == 1 ? 4 : 1)>(a, lds, st);
+    if (qg == 2)
+      return launch_tiled_nch<Real, TT, PP, (PP <= 2 ? 2 : 1)>(a, lds, st);
+    return launch_tiled_nch<Real, TT, PP, 1>(a, lds, st);
+  };
+  using I256 = std::integral_constant<int, 256>;
+  using I1024 = std::integral_constant<int, 1024>;
+  if (ts.threads == 256)
+    return ppt == 1 ? by_qg(I256{}, std::integral_constant<int, 1>{})
+                    : by_qg(I256{}, std::integral_constant<int, 2>{});
   switch (ppt) {
-    G4_TILED_CASE(1)
-    G4_TILED_CASE(2)
-    G4_TILED_CASE(4)
-    G4_TILED_CASE(6)
-    G4_TILED_CASE(8)
-    G4_TILED_CASE(16)
-    default:
-      return false;
+    case 1: return by_qg(I1024{}, std::integral_constant<int, 1>{});
+    case 2: return by_qg(I1024{}, std::integral_constant<int, 2>{});
+    case 4: return by_qg(I1024{}, std::integral_constant<int, 4>{});
+    default: return false;
   }
-#undef G4_TILED_CASE
 }
 
 template bool launch_g4_tiled<double>(TiledArgs<double>, hipStream_t);

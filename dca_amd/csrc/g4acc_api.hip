@@ -65,6 +65,8 @@ struct g4acc_handle {
   bool staged_pending = false;
   // Captured launch sequences, one per batch size (G4ACC_GRAPH=0 disables them).
   bool use_graph = true;
+  int mtog_tw = 0;                 // tuning: frequencies per thread of the fused M -> G kernel
+  int g4_qg = 0;                   // tuning: momenta per G4 workgroup (0: register limit)
   std::vector<std::pair<int, hipGraphExec_t>> graphs;
   // stage timing (profiling hook)
   bool timing = false;
@@ -247,6 +249,7 @@ MtoGArgs<Real> mtog_args(g4acc_handle* h, int n_slots) {
   a.n_max = h->max_vertices;
   a.chunk = h->chunk;
   a.fused = h->fused_mtog;
+  a.tw = h->mtog_tw;
   a.perm = h->d_perm;
   a.off = h->d_off;
   a.T = static_cast<Cplx<Real>*>(h->d_T);
@@ -289,6 +292,7 @@ void run_g4(g4acc_handle* h, int n_walkers) {
     a.k_ex = h->d_kex;
     a.w_ex = h->d_wex;
     a.nqg = 1;
+    a.qg_cap = h->g4_qg;
     // XCD-aware super-tiles pay off when many walkers re-read G; a single walker's launch is
     // dominated by the G4 read-modify-write, which the natural order streams best.
     a.tile_order = h->tile_order >= 0 ? h->tile_order : (n_walkers > 1 ? 1 : 0);
@@ -509,7 +513,7 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
     int max_lds = 0;
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess)
       max_lds = 65536;
-    const size_t lds = tiled_lds_bytes(g, need_y, need_z, cb) + cb * 256;
+    const size_t lds = tiled_lds_bytes(g, need_y, need_z, cb) + cb * 4 * 4 * g.nc;
     if (const char* to = getenv("G4ACC_TILE_ORDER"))
       h->tile_order = atoi(to);
     if (const char* gr = getenv("G4ACC_GRAPH"))
@@ -771,6 +775,28 @@ int g4acc_accumulate_G(g4acc_handle* h, const double* G_ref, double sign) {
   }
   (void)hipFree(tmp);
   return rc;
+}
+
+int g4acc_set_tuning(g4acc_handle* h, const char* key, int value) {
+  if (!h || !key)
+    return G4ACC_INVALID_ARG;
+  const std::string k(key);
+  if (k == "mtog_tw" && (value == 0 || value == 1 || value == 2 || value == 4 || value == 8))
+    h->mtog_tw = value;
+  else if (k == "g4_qg" && (value == 0 || value == 1 || value == 2 || value == 4))
+    h->g4_qg = value;
+  else if (k == "g4_tile_order" && value >= -1 && value <= 1)
+    h->tile_order = value;
+  else if (k == "graph" && (value == 0 || value == 1))
+    h->use_graph = value != 0;
+  else
+    return h->fail(G4ACC_INVALID_ARG, "unknown tuning key or value: " + k);
+  // Captured sequences bake the launch configuration in: drop them.
+  G4_TRY(h->check(hipStreamSynchronize(h->stream), "hipStreamSynchronize"));
+  for (auto& e : h->graphs)
+    (void)hipGraphExecDestroy(e.second);
+  h->graphs.clear();
+  return G4ACC_OK;
 }
 
 int g4acc_set_timing(g4acc_handle* h, int enable) {

@@ -13,6 +13,7 @@ struct MtoGArgs {
   int n_max;              // vertex capacity per slot
   int chunk;              // slots whose S1..S4 workspaces exist at once
   bool fused;             // use the register-fused single-band kernels
+  int tw;                 // frequencies per thread in the fused right kernel (0: default)
   const int* perm;        // [n_slots][n_max]
   const int* off;         // [n_slots][n_orb + 1]
   Cplx<Real>* T;          // [n_slots][n_max][nwE]
@@ -73,6 +74,7 @@ struct TiledArgs {
   const int* w_ex;
   int nqg;                     // momentum groups per exchange frequency (set by the launcher)
   int tile_order;              // 0: natural, 1: XCD-aware super-tiles
+  int qg_cap;                  // largest momentum group (0: as many as registers allow)
   int super_tile, n_super, n_xcd;  // tile schedule (set by the launcher)
   int sum_mode;
   bool need_y, need_z, has_trans;

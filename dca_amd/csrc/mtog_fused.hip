@@ -208,20 +208,27 @@ bool launch_mtog_fused(const MtoGArgs<Real>& a, hipStream_t st) {
     }
     return true;
   };
+  const int tw = a.tw;
+#define G4_RUN(NC_, TW_) run([&](int s0, int ns) { launch_fused_nc<Real, NC_, TW_>(a, s0, ns, st); })
   switch (g.nc) {
     case 4:
-      return run([&](int s0, int ns) { launch_fused_nc<Real, 4, 8>(a, s0, ns, st); });
+      return tw == 4 ? G4_RUN(4, 4) : G4_RUN(4, 8);
     case 8:
-      return run([&](int s0, int ns) { launch_fused_nc<Real, 8, (dp ? 4 : 8)>(a, s0, ns, st); });
+      if (tw == 2)
+        return G4_RUN(8, 2);
+      return (dp || tw == 4) ? G4_RUN(8, 4) : G4_RUN(8, 8);
     case 16:
-      return run([&](int s0, int ns) { launch_fused_nc<Real, 16, (dp ? 2 : 4)>(a, s0, ns, st); });
+      if (tw == 1)
+        return G4_RUN(16, 1);
+      return (dp || tw == 2) ? G4_RUN(16, 2) : G4_RUN(16, 4);
     case 36:
-      return run([&](int s0, int ns) { launch_fused_nc<Real, 36, 1>(a, s0, ns, st); });
+      return (dp || tw != 2) ? G4_RUN(36, 1) : G4_RUN(36, 2);
     case 64:
-      return run([&](int s0, int ns) { launch_fused_nc<Real, 64, 1>(a, s0, ns, st); });
+      return G4_RUN(64, 1);
     default:
       return false;
   }
+#undef G4_RUN
 }
 
 bool mtog_fused_covers(const Geo& g) {

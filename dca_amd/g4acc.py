@@ -31,7 +31,7 @@ EXPORTED = [
     "g4acc_create", "g4acc_destroy", "g4acc_reset", "g4acc_accumulate", "g4acc_accumulate_batch",
     "g4acc_finalize", "g4acc_sync", "g4acc_get_g4", "g4acc_sum_to", "g4acc_g4_size", "g4acc_bytes",
     "g4acc_stream", "g4acc_last_error", "g4acc_compute_G", "g4acc_g_size", "g4acc_accumulate_G",
-    "g4acc_set_timing", "g4acc_get_timing",
+    "g4acc_set_timing", "g4acc_get_timing", "g4acc_set_tuning",
 ]
 
 
@@ -94,6 +94,7 @@ def load_library(path: str = LIB_PATH):
         "g4acc_g_size": (sz, [vp]),
         "g4acc_accumulate_G": (i32, [vp, vp, ctypes.c_double]),
         "g4acc_set_timing": (i32, [vp, i32]),
+        "g4acc_set_tuning": (i32, [vp, ctypes.c_char_p, i32]),
         "g4acc_get_timing": (i32, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_longlong)]),
     }
@@ -161,7 +162,7 @@ class TpAccumulator:
     """g4acc-backed TpAccumulator (one HIP device, one stream)."""
 
     def __init__(self, geom: geo.Geometry, precision: str = "fp64", sum_mode: str = "fp64",
-                 max_walkers: int = 1, max_vertices: int = 64, device: int = 0):
+                 max_walkers: int = 1, max_vertices: int = 64, device: int = 0, tuned: bool = True):
         self.lib = load_library()
         self.geom = geom
         self._keep = _Keep()
@@ -197,6 +198,13 @@ class TpAccumulator:
             _raise(rc, self.lib.g4acc_last_error(None).decode())
         self._h = h
         self._inflight = []
+        self.tuning = None
+        if tuned:
+            # Launch shapes measured by dca_amd/autotune.py for this geometry and batch size.
+            from . import autotune
+            self.tuning = autotune.lookup(geom, precision, self.max_walkers)
+            if self.tuning:
+                self.set_tuning(**self.tuning)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -280,6 +288,11 @@ class TpAccumulator:
     @property
     def device_bytes(self) -> int:
         return self.lib.g4acc_bytes(self._h)
+
+    def set_tuning(self, **params):
+        """Launch-shape tuning: mtog_tw, g4_qg, g4_tile_order, graph (see include/g4acc.h)."""
+        for k, v in params.items():
+            self._check(self.lib.g4acc_set_tuning(self._h, k.encode(), int(v)))
 
     def set_timing(self, enable: bool):
         self._check(self.lib.g4acc_set_timing(self._h, 1 if enable else 0))

@@ -153,6 +153,14 @@ int g4acc_accumulate_G(g4acc_handle* h, const double* G, double sign);
  * tp_accumulator_cpu.hpp:236,416): when enabled, every batch records HIP events around its
  * M -> G and G4 stages and waits for them; times accumulate until the next enable call. */
 int g4acc_set_timing(g4acc_handle* h, int enable);
+
+/* Launch-shape tuning (no counterpart in the reference, whose launch shapes are fixed; set by
+ * the autotuner's table, dca_amd/autotune.py). Keys: "mtog_tw" (frequencies per thread of the
+ * fused single-band M -> G kernel: 0 default, 1, 2, 4, 8), "g4_qg" (momenta per G4 workgroup:
+ * 0 = as many as registers allow, 1, 2, 4), "g4_tile_order" (-1 by batch size, 0 natural,
+ * 1 XCD-aware super-tiles), "graph" (1: replay captured batch sequences, 0: launch eagerly).
+ * Results do not depend on the tuning beyond FP32 summation order. */
+int g4acc_set_tuning(g4acc_handle* h, const char* key, int value);
 int g4acc_get_timing(const g4acc_handle* h, double* mtog_ms, double* g4_ms, long long* batches);
 
 #ifdef __cplusplus
