@@ -4,7 +4,8 @@ For a workload and a batch size it measures, with the library's own HIP-event st
 its stream, every candidate of each tunable of include/g4acc.h (g4acc_set_tuning):
   mtog_tw        frequencies per thread of the fused single-band M -> G kernel (M -> G stage)
   g4_qg          momenta per G4 workgroup                                        (G4 stage)
-  g4_tile_order  natural or XCD-aware super-tile schedule of the G4 kernel       (G4 stage)
+  g4_tile_order  natural, XCD-aware super-tile or diagonal-strip tile schedule   (G4 stage)
+  g4_mfma        rank-one G4 terms on the matrix cores or the vector ALUs         (G4 stage)
 The stages are independent, so each is tuned on its own (the G4 knobs jointly). Results go to a
 JSON table (dca_amd/tuning.json by default) keyed by the geometry, the precision and the batch
 size; TpAccumulator applies the matching entry when it is constructed.
@@ -26,7 +27,8 @@ TABLE = os.path.join(HERE, "tuning.json")
 CANDIDATES = {
     "mtog_tw": [0, 1, 2, 4, 8],
     "g4_qg": [0, 1, 2, 4],
-    "g4_tile_order": [0, 1],
+    "g4_tile_order": [0, 1, 2],
+    "g4_mfma": [0, 1],
 }
 
 
@@ -83,16 +85,18 @@ def tune(cfg: int, walkers: int, precision: str, reps: int = 4, log=print) -> di
         if t < best_t:
             best_tw, best_t = tw, t
     acc.set_tuning(mtog_tw=best_tw)
-    best_g4, best_g4_t = (0, 0), float("inf")
-    for qg, order in itertools.product(CANDIDATES["g4_qg"], CANDIDATES["g4_tile_order"]):
-        acc.set_tuning(g4_qg=qg, g4_tile_order=order)
+    best_g4, best_g4_t = (0, 0, 1), float("inf")
+    for qg, order, mf in itertools.product(CANDIDATES["g4_qg"], CANDIDATES["g4_tile_order"],
+                                           CANDIDATES["g4_mfma"]):
+        acc.set_tuning(g4_qg=qg, g4_tile_order=order, g4_mfma=mf)
         _, t = _stage_times(acc, batches, reps)
-        measured["g4"][f"g4_qg={qg},g4_tile_order={order}"] = round(t, 4)
-        log(f"  G4    g4_qg={qg} g4_tile_order={order}: {t:.3f} ms")
+        measured["g4"][f"g4_qg={qg},g4_tile_order={order},g4_mfma={mf}"] = round(t, 4)
+        log(f"  G4    g4_qg={qg} g4_tile_order={order} g4_mfma={mf}: {t:.3f} ms")
         if t < best_g4_t:
-            best_g4, best_g4_t = (qg, order), t
+            best_g4, best_g4_t = (qg, order, mf), t
     acc.close()
-    return {"params": {"mtog_tw": best_tw, "g4_qg": best_g4[0], "g4_tile_order": best_g4[1]},
+    return {"params": {"mtog_tw": best_tw, "g4_qg": best_g4[0], "g4_tile_order": best_g4[1],
+                       "g4_mfma": best_g4[2]},
             "stage_ms": {"mtog": round(best_t, 4), "g4": round(best_g4_t, 4)},
             "measured": measured, "workload": cfg, "when": time.strftime("%Y-%m-%d %H:%M:%S")}
 

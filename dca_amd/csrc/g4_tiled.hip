@@ -17,39 +17,13 @@
 // Products are formed in the G precision and summed per term type over 8 walkers before they are
 // added to FP64 accumulators; G4 is read and written once per launch (FP64, or FP32 value + Kahan
 // compensation).
+#include <algorithm>
 #include <type_traits>
+#include <utility>
 
-#include "g4acc_common.hpp"
-#include "g4acc_kernels.hpp"
+#include "g4_tile_common.hpp"
 
 namespace g4acc {
-
-namespace {
-constexpr int PH_MAGNETIC = 2, PH_CHARGE = 3, PP_UP_DOWN = 4;  // else: PH_TRANSVERSE (1)
-
-__device__ __forceinline__ void g4_store_t(void* g4, size_t i, double2 acc, int sum_mode) {
-  if (sum_mode == SUM_FP64) {
-    double2* p = static_cast<double2*>(g4) + i;
-    double2 v = *p;
-    v.x += acc.x;
-    v.y += acc.y;
-    *p = v;
-  }
-  else {
-    float4* p = static_cast<float4*>(g4) + i;
-    float4 v = *p;
-    const float hx = static_cast<float>(acc.x), hy = static_cast<float>(acc.y);
-    const float lx = static_cast<float>(acc.x - hx), ly = static_cast<float>(acc.y - hy);
-    const float yx = hx - v.z, yy = hy - v.w;
-    const float tx = v.x + yx, ty = v.y + yy;
-    v.z = ((tx - v.x) - yx) - lx;
-    v.w = ((ty - v.y) - yy) - ly;
-    v.x = tx;
-    v.y = ty;
-    *p = v;
-  }
-}
-}  // namespace
 
 // UV[((m * 4 + plane) * nqv + qv) * P + p], planes: a(u0-u1), a(u0+u1), v0-v1, v0+v1.
 template <typename Real>
@@ -90,29 +64,8 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
   const Geo& g = a.g;
   const int nc = g.nc, nw = g.nw, e = g.ext, P = nc * nw;
   const size_t D = g.D, DD = D * D;
-  const int nz = g.nv * a.nqg;
   int w1, w2, z;
-  if (a.n_xcd > 0) {
-    // Workgroups are dealt to the XCDs round-robin by id; give each XCD a contiguous range of
-    // logical tiles, ordered by super-tiles of ST x ST (w1, w2) blocks with the exchange
-    // frequency and momentum group innermost.
-    const int ST = a.super_tile, nsw = a.n_super;
-    const unsigned n_tiles = gridDim.x;
-    const unsigned xcd = blockIdx.x % a.n_xcd, local = blockIdx.x / a.n_xcd;
-    const unsigned L = xcd * (n_tiles / a.n_xcd) + local;
-    const unsigned per_super = static_cast<unsigned>(ST) * ST * nz;
-    const unsigned sup = L / per_super, rem = L % per_super;
-    z = rem % nz;
-    w1 = (sup % nsw) * ST + (rem / nz) % ST;
-    w2 = (sup / nsw) * ST + (rem / nz) / ST;
-  }
-  else {
-    // Natural order: w1 fastest, then w2, then (v, momentum group).
-    w1 = blockIdx.x % nw;
-    w2 = (blockIdx.x / nw) % nw;
-    z = blockIdx.x / (nw * nw);
-  }
-  if (w1 >= nw || w2 >= nw)
+  if (!tile_coords(a, w1, w2, z))
     return;
   const int iv = z / a.nqg, q0 = (z % a.nqg) * QG;
   const int nu = a.w_ex[iv];
@@ -396,19 +349,7 @@ namespace {
 
 template <typename Real, int TT, int PPT, int QG, int NCH, int MASK>
 void launch_tiled_fixed(TiledArgs<Real> a, size_t lds, hipStream_t st) {
-  const Geo& g = a.g;
-  unsigned tiles;
-  if (a.tile_order == 1) {
-    a.super_tile = g.nw < 8 ? g.nw : 8;
-    a.n_super = (g.nw + a.super_tile - 1) / a.super_tile;
-    tiles = static_cast<unsigned>(a.n_super * a.super_tile) * (a.n_super * a.super_tile) * g.nv * a.nqg;
-    a.n_xcd = tiles % 8 == 0 ? 8 : 1;
-  }
-  else {
-    a.super_tile = a.n_super = 1;
-    a.n_xcd = 0;
-    tiles = static_cast<unsigned>(g.nw) * g.nw * g.nv * a.nqg;
-  }
+  const unsigned tiles = tile_schedule(a);
   hipLaunchKernelGGL((g4_tiled_kernel<Real, TT, PPT, QG, NCH, MASK>), dim3(tiles), dim3(TT), lds, st, a);
 }
 
@@ -460,9 +401,43 @@ TileShape tile_shape(int nc) {
 
 int tiled_pairs_per_thread(int nc) { return tile_shape(nc).ppt; }
 
-size_t tiled_lds_bytes(const Geo& g, bool need_y, bool need_z, size_t cplx_bytes) {
+// Tile (w1, w2) of G4 reads the G blocks X = (w1, w2), Y = (w2 + v, w1 + v) and
+// Z = (v + nw - 1 - w1, v + nw - 1 - w2) (extended frequencies). The transpose of a tile holds its
+// Y block as X, its mirror (w -> nw - 1 - w) its Z block, and the neighbour along the diagonal
+// the next v. So a group is `strip` consecutive positions on the diagonals w1 - w2 = +d and -d
+// together with their mirror images; the workgroups resident on one XCD at a time cover a few
+// groups and find most of their blocks in its L2.
+std::vector<int> diagonal_tile_order(int nw, int strip) {
+  std::vector<int> out;
+  out.reserve(static_cast<size_t>(nw) * nw);
+  std::vector<char> seen(static_cast<size_t>(nw) * nw, 0);
+  for (int d = 0; d < nw; ++d)
+    for (int p0 = 0; p0 < nw - d; p0 += strip)
+      for (int sgn = 0; sgn < 2; ++sgn) {
+        if (d == 0 && sgn == 1)
+          continue;
+        for (int p = p0; p < std::min(p0 + strip, nw - d); ++p)
+          for (int mir = 0; mir < 2; ++mir) {
+            int w2 = p, w1 = p + d;
+            if (sgn)
+              std::swap(w1, w2);
+            if (mir) {
+              w1 = nw - 1 - w1;
+              w2 = nw - 1 - w2;
+            }
+            char& s = seen[w1 + static_cast<size_t>(nw) * w2];
+            if (s)
+              continue;
+            s = 1;
+            out.push_back(w1 | (w2 << 16));
+          }
+      }
+  return out;
+}
+
+size_t tiled_lds_bytes(const Geo& g, bool need_y, bool need_z, size_t cplx_bytes, int qg) {
   const size_t bs = static_cast<size_t>(g.nc) * (g.nc + 1);
-  return cplx_bytes * bs * 2 * ((need_y ? 1 : 0) + (need_z ? 1 : 0));
+  return cplx_bytes * (bs * 2 * ((need_y ? 1 : 0) + (need_z ? 1 : 0)) + 4 * static_cast<size_t>(qg) * g.nc);
 }
 
 template <typename Real>
@@ -488,10 +463,13 @@ bool launch_g4_tiled(TiledArgs<Real> a, hipStream_t st) {
   if (ppt * qg * a.n_ch > 16 || 4 * qg * g.nc > ts.threads)
     return false;  // accumulators would not fit in registers / one staged factor per thread
   a.nqg = (g.nq + qg - 1) / qg;
-  const size_t lds = tiled_lds_bytes(g, a.need_y, a.need_z, sizeof(Cplx<Real>)) +
-                     sizeof(Cplx<Real>) * 4 * qg * g.nc;
+  const size_t lds = tiled_lds_bytes(g, a.need_y, a.need_z, sizeof(Cplx<Real>), qg);
   const dim3 ug((g.nc * g.nw + 255) / 256, g.nq * g.nv, a.n_walkers);
   hipLaunchKernelGGL(uv_kernel<Real>, ug, dim3(256), 0, st, a);
+  if constexpr (std::is_same<Real, float>::value) {
+    if (a.mfma && ts.threads == 256 && ppt == 1 && launch_g4_mfma(a, qg, st))
+      return true;
+  }
   auto by_qg = [&](auto tt, auto pp) {
     constexpr int TT = decltype(tt)::value, PP = decltype(pp)::value;
     if (qg == 4)

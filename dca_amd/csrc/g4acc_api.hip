@@ -44,6 +44,7 @@ struct g4acc_handle {
   std::vector<void*> allocations;
   size_t bytes = 0;
   int *d_kadd = nullptr, *d_ksub = nullptr, *d_kex = nullptr, *d_wex = nullptr;
+  int* d_tperm = nullptr;          // G4 tile schedule 2 (diagonal strips)
   void* d_dft = nullptr;           // Cplx<Real>[Nc*Nc]
   double2* d_bphase = nullptr;     // [nb*Nc] or null
   double2* d_G0 = nullptr;
@@ -67,10 +68,11 @@ struct g4acc_handle {
   bool use_graph = true;
   int mtog_tw = 0;                 // tuning: frequencies per thread of the fused M -> G kernel
   int g4_qg = 0;                   // tuning: momenta per G4 workgroup (0: register limit)
-  std::vector<std::pair<int, hipGraphExec_t>> graphs;
+  int g4_mfma = 1;                 // tuning: rank-one G4 terms on the matrix cores (FP32 G, Nc <= 16)
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;  // key: (walkers << 1) | stage
   // stage timing (profiling hook)
   bool timing = false;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double t_mtog_ms = 0, t_g4_ms = 0;
   long long timed_batches = 0;
 
@@ -167,7 +169,8 @@ void upload_dft(std::vector<unsigned char>& buf, const double* dft, int nc) {
 
 // Resolves the walkers of one chunk into slot descriptors; host inputs are packed into the
 // pinned staging buffer and copied once. Returns the number of slots.
-int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int memory) {
+int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int memory,
+                  hipStream_t st) {
   const int nb = h->g.nb, nc = h->g.nc;
   // The pinned descriptor and staging buffers are reused: wait until the previous copies are done.
   if (h->staged_pending) {
@@ -229,13 +232,13 @@ int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int m
   const int n_slots = 2 * nw_;
   if (of)
     G4_TRY(h->check(hipMemcpyAsync(h->d_stage_f, h->h_stage_f, of * sizeof(double),
-                                   hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync(M, tau)"));
+                                   hipMemcpyHostToDevice, st), "hipMemcpyAsync(M, tau)"));
   if (oi)
     G4_TRY(h->check(hipMemcpyAsync(h->d_stage_i, h->h_stage_i, oi * sizeof(int),
-                                   hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync(config)"));
+                                   hipMemcpyHostToDevice, st), "hipMemcpyAsync(config)"));
   G4_TRY(h->check(hipMemcpyAsync(h->d_desc, h->h_desc, sizeof(SlotDesc) * n_slots,
-                                 hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync(desc)"));
-  G4_TRY(h->check(hipEventRecord(h->staged, h->stream), "hipEventRecord"));
+                                 hipMemcpyHostToDevice, st), "hipMemcpyAsync(desc)"));
+  G4_TRY(h->check(hipEventRecord(h->staged, st), "hipEventRecord"));
   h->staged_pending = true;
   return G4ACC_OK;
 }
@@ -264,9 +267,9 @@ MtoGArgs<Real> mtog_args(g4acc_handle* h, int n_slots) {
 }
 
 template <typename Real>
-void run_mtog(g4acc_handle* h, int n_slots) {
-  launch_sort(h->d_desc, h->g, n_slots, h->max_vertices, h->d_perm, h->d_off, h->d_err, h->stream);
-  launch_mtog<Real>(mtog_args<Real>(h, n_slots), h->stream);
+void run_mtog(g4acc_handle* h, int n_slots, hipStream_t st) {
+  launch_sort(h->d_desc, h->g, n_slots, h->max_vertices, h->d_perm, h->d_off, h->d_err, st);
+  launch_mtog<Real>(mtog_args<Real>(h, n_slots), st);
 }
 
 template <typename Real>
@@ -296,7 +299,9 @@ void run_g4(g4acc_handle* h, int n_walkers) {
     // XCD-aware super-tiles pay off when many walkers re-read G; a single walker's launch is
     // dominated by the G4 read-modify-write, which the natural order streams best.
     a.tile_order = h->tile_order >= 0 ? h->tile_order : (n_walkers > 1 ? 1 : 0);
+    a.tile_perm = h->d_tperm;
     a.sum_mode = h->sum_mode;
+    a.mfma = h->g4_mfma;
     if (launch_g4_tiled<Real>(a, h->stream))
       return;
   }
@@ -317,33 +322,35 @@ void run_g4(g4acc_handle* h, int n_walkers) {
   }
 }
 
-void launch_sequence(g4acc_handle* h, int n_walkers) {
-  if (h->precision == G4ACC_FP64) {
-    run_mtog<double>(h, 2 * n_walkers);
+void launch_stage(g4acc_handle* h, int n_walkers, int stage, hipStream_t st) {
+  if (stage == 0) {
+    if (h->precision == G4ACC_FP64)
+      run_mtog<double>(h, 2 * n_walkers, st);
+    else
+      run_mtog<float>(h, 2 * n_walkers, st);
+  }
+  else if (h->precision == G4ACC_FP64)
     run_g4<double>(h, n_walkers);
-  }
-  else {
-    run_mtog<float>(h, 2 * n_walkers);
+  else
     run_g4<float>(h, n_walkers);
-  }
 }
 
-// The kernels of one batch read everything batch-specific (vertex counts, pointers, signs) from
-// the slot descriptors, so the sequence for a given batch size is captured once into a hipGraph
-// and replayed; only the descriptor copy precedes each replay. The sequence holds kernel launches
-// only (no memset nodes).
-int run_batch_graph(g4acc_handle* h, int n_walkers) {
+// The kernels of one stage read everything batch-specific (vertex counts, pointers, signs) from
+// the slot descriptors, so each stage is captured once per batch size into a hipGraph and
+// replayed; only the descriptor copy precedes each replay. The captured sequences hold kernel
+// launches only (no memset nodes).
+int run_stage_graph(g4acc_handle* h, int n_walkers, int stage, hipStream_t st) {
+  const int key = (n_walkers << 1) | stage;
   hipGraphExec_t exec = nullptr;
   for (auto& e : h->graphs)
-    if (e.first == n_walkers)
+    if (e.first == key)
       exec = e.second;
   if (!exec) {
     hipGraph_t graph = nullptr;
-    G4_TRY(h->check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal),
-                    "hipStreamBeginCapture"));
-    launch_sequence(h, n_walkers);
+    G4_TRY(h->check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"));
+    launch_stage(h, n_walkers, stage, st);
     const hipError_t launched = hipGetLastError();
-    const hipError_t ended = hipStreamEndCapture(h->stream, &graph);
+    const hipError_t ended = hipStreamEndCapture(st, &graph);
     if (launched != hipSuccess || ended != hipSuccess || !graph) {
       if (graph)
         (void)hipGraphDestroy(graph);
@@ -352,38 +359,47 @@ int run_batch_graph(g4acc_handle* h, int n_walkers) {
     const hipError_t inst = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     G4_TRY(h->check(inst, "hipGraphInstantiate"));
-    h->graphs.emplace_back(n_walkers, exec);
+    h->graphs.emplace_back(key, exec);
   }
-  return h->check(hipGraphLaunch(exec, h->stream), "hipGraphLaunch");
+  return h->check(hipGraphLaunch(exec, st), "hipGraphLaunch");
 }
 
-int run_batch(g4acc_handle* h, int n_walkers) {
-  const int n_slots = 2 * n_walkers;
+int run_stage(g4acc_handle* h, int n_walkers, int stage, hipStream_t st) {
   if (h->use_graph && !h->timing)
-    return run_batch_graph(h, n_walkers);
+    return run_stage_graph(h, n_walkers, stage, st);
+  launch_stage(h, n_walkers, stage, st);
+  return h->check(hipGetLastError(), "kernel launch");
+}
+
+// Runs the staged batch: M -> G, then the G4 update, on the handle's stream.
+int run_batch(g4acc_handle* h, int n_walkers) {
+  hipStream_t st = h->stream;
   if (h->timing)
-    G4_TRY(h->check(hipEventRecord(h->ev[0], h->stream), "hipEventRecord"));
-  if (h->precision == G4ACC_FP64)
-    run_mtog<double>(h, n_slots);
-  else
-    run_mtog<float>(h, n_slots);
+    G4_TRY(h->check(hipEventRecord(h->ev[0], st), "hipEventRecord"));
+  G4_TRY(run_stage(h, n_walkers, 0, st));
   if (h->timing)
-    G4_TRY(h->check(hipEventRecord(h->ev[1], h->stream), "hipEventRecord"));
-  if (h->precision == G4ACC_FP64)
-    run_g4<double>(h, n_walkers);
-  else
-    run_g4<float>(h, n_walkers);
-  if (h->timing) {
+    G4_TRY(h->check(hipEventRecord(h->ev[1], st), "hipEventRecord"));
+  if (h->timing)
     G4_TRY(h->check(hipEventRecord(h->ev[2], h->stream), "hipEventRecord"));
-    G4_TRY(h->check(hipEventSynchronize(h->ev[2]), "hipEventSynchronize"));
+  G4_TRY(run_stage(h, n_walkers, 1, h->stream));
+  if (h->timing) {
+    G4_TRY(h->check(hipEventRecord(h->ev[3], h->stream), "hipEventRecord"));
+    G4_TRY(h->check(hipEventSynchronize(h->ev[3]), "hipEventSynchronize"));
     float a = 0, b = 0;
     G4_TRY(h->check(hipEventElapsedTime(&a, h->ev[0], h->ev[1]), "hipEventElapsedTime"));
-    G4_TRY(h->check(hipEventElapsedTime(&b, h->ev[1], h->ev[2]), "hipEventElapsedTime"));
+    G4_TRY(h->check(hipEventElapsedTime(&b, h->ev[2], h->ev[3]), "hipEventElapsedTime"));
     h->t_mtog_ms += a;
     h->t_g4_ms += b;
     ++h->timed_batches;
   }
-  return h->check(hipGetLastError(), "kernel launch");
+  return G4ACC_OK;
+}
+
+// Waits for all work of the handle.
+int quiesce(g4acc_handle* h) {
+  G4_TRY(h->check(hipStreamSynchronize(h->stream), "hipStreamSynchronize"));
+  h->staged_pending = false;
+  return G4ACC_OK;
 }
 
 int check_walker(g4acc_handle* h, const g4acc_walker& w) {
@@ -489,6 +505,7 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
 
   if ((rc = h->alloc(&h->d_kadd, nc2, "k_add")) || (rc = h->alloc(&h->d_ksub, nc2, "k_sub")) ||
       (rc = h->alloc(&h->d_kex, g.nq, "k_ex")) || (rc = h->alloc(&h->d_wex, g.nv, "w_ex")) ||
+      (rc = h->alloc(&h->d_tperm, static_cast<size_t>(g.nw) * g.nw, "tile order")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_dft), cb * nc2, "dft")) ||
       (rc = h->alloc(&h->d_G0, static_cast<size_t>(g.nb) * g.nb * 2 * g.nc * g.nwE, "G0")) ||
       (rc = h->alloc(&h->d_perm, W2 * NM, "perm")) ||
@@ -513,7 +530,7 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
     int max_lds = 0;
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess)
       max_lds = 65536;
-    const size_t lds = tiled_lds_bytes(g, need_y, need_z, cb) + cb * 4 * 4 * g.nc;
+    const size_t lds = tiled_lds_bytes(g, need_y, need_z, cb, 4);
     if (const char* to = getenv("G4ACC_TILE_ORDER"))
       h->tile_order = atoi(to);
     if (const char* gr = getenv("G4ACC_GRAPH"))
@@ -543,6 +560,7 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
     return bail(rc);
 
   // Constant tables.
+  const std::vector<int> tperm = diagonal_tile_order(g.nw, 2);
   std::vector<unsigned char> dftbuf;
   if (h->precision == G4ACC_FP64)
     upload_dft<double>(dftbuf, gm->dft, g.nc);
@@ -552,6 +570,8 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
       (rc = h->check(hipMemcpy(h->d_ksub, gm->k_sub, sizeof(int) * nc2, hipMemcpyHostToDevice), "copy")) ||
       (rc = h->check(hipMemcpy(h->d_kex, gm->k_ex, sizeof(int) * g.nq, hipMemcpyHostToDevice), "copy")) ||
       (rc = h->check(hipMemcpy(h->d_wex, gm->w_ex, sizeof(int) * g.nv, hipMemcpyHostToDevice), "copy")) ||
+      (rc = h->check(hipMemcpy(h->d_tperm, tperm.data(), sizeof(int) * tperm.size(), hipMemcpyHostToDevice),
+                     "copy")) ||
       (rc = h->check(hipMemcpy(h->d_dft, dftbuf.data(), dftbuf.size(), hipMemcpyHostToDevice), "copy")) ||
       (rc = h->check(hipMemcpy(h->d_G0, gm->g0,
                                sizeof(double2) * g.nb * g.nb * 2 * g.nc * g.nwE, hipMemcpyHostToDevice),
@@ -629,7 +649,7 @@ int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_w
   }
   for (size_t i0 = 0; i0 < todo.size(); i0 += h->max_walkers) {
     const int nw_ = static_cast<int>(std::min<size_t>(h->max_walkers, todo.size() - i0));
-    G4_TRY(stage_walkers(h, todo.data() + i0, nw_, memory));
+    G4_TRY(stage_walkers(h, todo.data() + i0, nw_, memory, h->stream));
     G4_TRY(run_batch(h, nw_));
   }
   return G4ACC_OK;
@@ -643,8 +663,7 @@ int g4acc_sync(g4acc_handle* h) {
   if (!h)
     return G4ACC_INVALID_ARG;
   G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
-  G4_TRY(h->check(hipStreamSynchronize(h->stream), "hipStreamSynchronize"));
-  h->staged_pending = false;
+  G4_TRY(quiesce(h));
   int flag = 0;
   G4_TRY(h->check(hipMemcpy(&flag, h->d_err, sizeof(int), hipMemcpyDeviceToHost), "read error flag"));
   if (flag) {
@@ -717,12 +736,13 @@ int g4acc_compute_G(g4acc_handle* h, const g4acc_walker* walker, int memory, dou
     return h->fail(G4ACC_INVALID_ARG, "invalid memory kind");
   G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
   G4_TRY(check_walker(h, *walker));
+  G4_TRY(quiesce(h));
   const g4acc_walker* ws[1] = {walker};
-  G4_TRY(stage_walkers(h, ws, 1, memory));
+  G4_TRY(stage_walkers(h, ws, 1, memory, h->stream));
   if (h->precision == G4ACC_FP64)
-    run_mtog<double>(h, 2);
+    run_mtog<double>(h, 2, h->stream);
   else
-    run_mtog<float>(h, 2);
+    run_mtog<float>(h, 2, h->stream);
   double2* tmp = nullptr;
   const size_t n = g4acc_g_size(h);
   G4_TRY(h->check(hipMalloc(&tmp, sizeof(double2) * n), "hipMalloc(export)"));
@@ -747,15 +767,12 @@ int g4acc_accumulate_G(g4acc_handle* h, const double* G_ref, double sign) {
   if (!h->g4_present)
     return G4ACC_OK;
   G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  G4_TRY(quiesce(h));
   const size_t n = g4acc_g_size(h);
   double2* tmp = nullptr;
   G4_TRY(h->check(hipMalloc(&tmp, sizeof(double2) * n), "hipMalloc(import)"));
   int rc = h->check(hipMemcpyAsync(tmp, G_ref, sizeof(double2) * n, hipMemcpyHostToDevice, h->stream),
                     "copy G");
-  if (rc == G4ACC_OK && h->staged_pending) {
-    rc = h->check(hipEventSynchronize(h->staged), "hipEventSynchronize");
-    h->staged_pending = false;
-  }
   if (rc == G4ACC_OK) {
     h->h_desc[0] = SlotDesc{nullptr, nullptr, nullptr, nullptr, 1, 1, sign};
     h->h_desc[1] = h->h_desc[0];
@@ -785,14 +802,16 @@ int g4acc_set_tuning(g4acc_handle* h, const char* key, int value) {
     h->mtog_tw = value;
   else if (k == "g4_qg" && (value == 0 || value == 1 || value == 2 || value == 4))
     h->g4_qg = value;
-  else if (k == "g4_tile_order" && value >= -1 && value <= 1)
+  else if (k == "g4_tile_order" && value >= -1 && value <= 2)
     h->tile_order = value;
+  else if (k == "g4_mfma" && (value == 0 || value == 1))
+    h->g4_mfma = value;
   else if (k == "graph" && (value == 0 || value == 1))
     h->use_graph = value != 0;
   else
     return h->fail(G4ACC_INVALID_ARG, "unknown tuning key or value: " + k);
   // Captured sequences bake the launch configuration in: drop them.
-  G4_TRY(h->check(hipStreamSynchronize(h->stream), "hipStreamSynchronize"));
+  G4_TRY(quiesce(h));
   for (auto& e : h->graphs)
     (void)hipGraphExecDestroy(e.second);
   h->graphs.clear();

@@ -76,14 +76,23 @@ struct TiledArgs {
   int tile_order;              // 0: natural, 1: XCD-aware super-tiles
   int qg_cap;                  // largest momentum group (0: as many as registers allow)
   int super_tile, n_super, n_xcd;  // tile schedule (set by the launcher)
+  const int* tile_perm;        // schedule 2: (w1 | w2 << 16) in dispatch order, nw * nw entries
   int sum_mode;
+  int mfma;                    // 1: rank-one terms on the matrix cores where a kernel provides it
   bool need_y, need_z, has_trans;
 };
 
 template <typename Real>
 bool launch_g4_tiled(TiledArgs<Real> a, hipStream_t st);
+// The matrix-core form of the single-band tile kernel (FP32 G, Nc <= 16, one pair per thread);
+// false if it does not cover the geometry or channel set. The caller has run uv_kernel and set
+// a.nqg for momentum groups of qg.
+bool launch_g4_mfma(const TiledArgs<float>& a, int qg, hipStream_t st);
 int tiled_pairs_per_thread(int nc);
-size_t tiled_lds_bytes(const Geo& g, bool need_y, bool need_z, size_t cplx_bytes);
+// Tile schedule 2: (w1 | w2 << 16) for all nw^2 frequency pairs, in dispatch order.
+std::vector<int> diagonal_tile_order(int nw, int strip);
+// LDS of one tile workgroup with momentum groups of qg (staged blocks and rank-one factors).
+size_t tiled_lds_bytes(const Geo& g, bool need_y, bool need_z, size_t cplx_bytes, int qg);
 
 // dst += src over n complex entries of a G4 buffer in the given sum mode.
 void launch_g4_add(void* dst, const void* src, size_t n, int sum_mode, hipStream_t st);

@@ -158,7 +158,9 @@ int g4acc_set_timing(g4acc_handle* h, int enable);
  * the autotuner's table, dca_amd/autotune.py). Keys: "mtog_tw" (frequencies per thread of the
  * fused single-band M -> G kernel: 0 default, 1, 2, 4, 8), "g4_qg" (momenta per G4 workgroup:
  * 0 = as many as registers allow, 1, 2, 4), "g4_tile_order" (-1 by batch size, 0 natural,
- * 1 XCD-aware super-tiles), "graph" (1: replay captured batch sequences, 0: launch eagerly).
+ * 1 XCD-aware super-tiles, 2 XCD-aware diagonal strips with their mirror images), "g4_mfma"
+ * (1: rank-one G4 terms on the matrix cores where available, FP32 G and Nc <= 16; 0: vector ALUs),
+ * "graph" (1: replay captured batch sequences, 0: launch eagerly).
  * Results do not depend on the tuning beyond FP32 summation order. */
 int g4acc_set_tuning(g4acc_handle* h, const char* key, int value);
 int g4acc_get_timing(const g4acc_handle* h, double* mtog_ms, double* g4_ms, long long* batches);

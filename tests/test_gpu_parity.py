@@ -213,6 +213,60 @@ def test_tiled_equals_generic_kernel(name, precision, monkeypatch):
         assert err < (1e-14 if precision == "fp64" else 5e-7)
 
 
+@pytest.mark.parametrize("name", ["1band_4x4_4q", "1band_6x6", "1band_8site"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_tile_schedules_agree(oracle_lib, name, precision):
+    """The three G4 tile schedules (natural, super-tiles, diagonal strips) cover every tile once:
+    same G4 up to summation order, and the oracle's."""
+    wl = case(name)
+    g = wl.geometry
+    ms = wl.draw(5)
+    ref = oracle_g4(oracle_lib, g, ms)
+    out = []
+    for order in (0, 1, 2):
+        acc = g4acc.TpAccumulator(g, precision=precision, max_walkers=5, max_vertices=wl.n_max,
+                                  tuned=False)
+        acc.set_tuning(g4_tile_order=order)
+        acc.resetAccumulation()
+        acc.accumulate_batch(ms)
+        out.append(acc.get_G4())
+        acc.close()
+    for c, r in enumerate(ref):
+        for order in (1, 2):
+            err = linf(out[0][c], out[order][c])
+            print(f"{name} {precision} order {order} vs natural l_inf={err:.3e}", flush=True)
+            assert err < (1e-14 if precision == "fp64" else 1e-7)
+        assert linf(r, out[2][c]) < (1e-12 if precision == "fp64" else 1e-6)
+
+
+MAG, CHG, PP = (geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE,
+                geo.PARTICLE_PARTICLE_UP_DOWN)
+
+
+@pytest.mark.parametrize("name", ["1band_4x4_4q", "1band_8site", "1band_3x3_shift2"])
+@pytest.mark.parametrize("channels", [(MAG, CHG, PP), (MAG,), (CHG, PP), (MAG, CHG)])
+def test_mfma_rank_one_matches_oracle(oracle_lib, name, channels):
+    """FP32 G4 with the rank-one terms on the matrix cores: against the FP64 oracle (1e-6) and
+    against the vector-ALU tile kernel (5e-7). Nine walkers leave a partial group of four and a
+    partial FP32 flush of eight."""
+    wl = case(name, channels=list(channels))
+    g = wl.geometry
+    ms = wl.draw(9)
+    ref = oracle_g4(oracle_lib, g, ms)
+    mf = g4acc.TpAccumulator(g, precision="fp32", max_walkers=9, max_vertices=wl.n_max, tuned=False)
+    vec = g4acc.TpAccumulator(g, precision="fp32", max_walkers=9, max_vertices=wl.n_max, tuned=False)
+    vec.set_tuning(g4_mfma=0)
+    for acc in (mf, vec):
+        acc.resetAccumulation()
+        acc.accumulate_batch(ms)
+    for ch, r, a, b in zip(g.channels, ref, mf.get_G4(), vec.get_G4()):
+        e_ref, e_vec = linf(r, a), linf(b, a)
+        print(f"{name} channel {ch}: mfma vs oracle {e_ref:.3e}, vs vector kernel {e_vec:.3e}",
+              flush=True)
+        assert e_ref < 1e-6
+        assert e_vec < 5e-7
+
+
 @pytest.mark.parametrize("name", ["1band_2x2", "1band_4x4_4q", "1band_6x6", "1band_8site"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_fused_equals_unfused_mtog(name, precision, monkeypatch):
