@@ -68,7 +68,7 @@ struct g4acc_handle {
   bool use_graph = true;
   int mtog_tw = 0;                 // tuning: frequencies per thread of the fused M -> G kernel
   int g4_qg = 0;                   // tuning: momenta per G4 workgroup (0: register limit)
-  int g4_mfma = 1;                 // tuning: rank-one G4 terms on the matrix cores (FP32 G, Nc <= 16)
+  int g4_mfma = 0;                 // tuning: rank-one G4 terms on the matrix cores (FP32 G, Nc <= 16)
   std::vector<std::pair<int, hipGraphExec_t>> graphs;  // key: (walkers << 1) | stage
   // stage timing (profiling hook)
   bool timing = false;

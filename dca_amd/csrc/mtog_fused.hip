@@ -59,9 +59,11 @@ __global__ void __launch_bounds__(256) mtog_left_fused(const SlotDesc* __restric
   }
 }
 
-// The phases of the workgroup's 4 TW frequencies (for a chunk of JC sorted vertices), the
-// cluster DFT matrix and the G0 slice of those frequencies are staged in LDS and read as
-// broadcasts; A rows are read from global memory, four in flight per step.
+// Per chunk of JV sorted vertices the workgroup stages in LDS the A rows of its 64 p1 (all four
+// waves read the same rows) and the phases of its 4 TW frequencies; both are loaded
+// cooperatively, eight A elements per thread in flight, and the next chunk is loaded into
+// registers while the current one is consumed. The cluster DFT matrix and the G0 slice of the
+// workgroup's frequencies are staged once.
 template <typename Real, int NC, int TW>
 __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restrict__ desc, Geo g,
                                                         int n_max, int slot0,
@@ -73,9 +75,12 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
                                                         const double2* __restrict__ G0,
                                                         Cplx<Real>* __restrict__ G) {
   using C = Cplx<Real>;
-  constexpr int WT = 4 * TW;     // frequencies per workgroup
-  constexpr int JC = 4096 / WT;  // vertices per staged phase chunk (4096 staged phases)
-  __shared__ C Tl[JC * WT];
+  constexpr int WT = 4 * TW;                  // frequencies per workgroup
+  constexpr int JV = 16;                      // vertices per staged chunk
+  constexpr int NA = JV * 64 / 256;           // A elements loaded per thread and chunk
+  constexpr int NT = (JV * WT + 255) / 256;   // phases loaded per thread and chunk
+  __shared__ C Al[JV * 64];
+  __shared__ C Tl[JV * WT];
   __shared__ C Dl[NC * NC];
   __shared__ double2 G0l[NC * WT];
   const int slot = slot0 + blockIdx.z;
@@ -89,7 +94,8 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
   const bool active = p1 < g.D && w2b < g.nwE;
   const int* O = off + static_cast<size_t>(slot) * (NC + 1);
   const C* Ts = T + static_cast<size_t>(slot) * g.nwE * n_max;
-  const C* As = A + static_cast<size_t>(blockIdx.z) * g.D * n_max + (p1 < g.D ? p1 : 0);
+  const C* Ab = A + static_cast<size_t>(blockIdx.z) * g.D * n_max + static_cast<size_t>(blockIdx.x) * 64;
+  const int rows = g.D - blockIdx.x * 64;     // valid p1 rows of the workgroup
 
   for (int i = threadIdx.x; i < NC * NC; i += 256)
     Dl[i] = dft[i];
@@ -98,6 +104,21 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
     G0l[i] = w < g.nwE ? G0[s + 2 * (k + static_cast<size_t>(NC) * w)] : double2{0, 0};
   }
 
+  C pa[NA], pt[NT];
+  auto prefetch = [&](int c0) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int i = threadIdx.x + 256 * u, jl = i / 64, pl = i % 64;
+      pa[u] = (c0 + jl < n && pl < rows) ? Ab[pl + static_cast<size_t>(g.D) * (c0 + jl)] : C{0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      const int i = threadIdx.x + 256 * u, wl = i % WT, jl = i / WT, w = w2wg + wl;
+      pt[u] = (i < JV * WT && c0 + jl < n && w < g.nwE) ? Ts[w + static_cast<size_t>(g.nwE) * (c0 + jl)]
+                                                       : C{0, 0};
+    }
+  };
+
   C e[TW][NC];
 #pragma unroll
   for (int tt = 0; tt < TW; ++tt)
@@ -105,37 +126,30 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
     for (int r = 0; r < NC; ++r)
       e[tt][r] = C{0, 0};
   const int tw_ok = g.nwE - w2b;  // frequencies of this wave inside the domain
-  for (int c0 = 0; c0 < n; c0 += JC) {
-    const int c1 = c0 + JC < n ? c0 + JC : n;
+  if (n > 0)
+    prefetch(0);
+  for (int c0 = 0; c0 < n; c0 += JV) {
+    const int c1 = c0 + JV < n ? c0 + JV : n;
     __syncthreads();
-    for (int i = threadIdx.x; i < (c1 - c0) * WT; i += 256) {
-      const int wl = i % WT, jl = i / WT, w = w2wg + wl;
-      Tl[i] = w < g.nwE ? Ts[w + static_cast<size_t>(g.nwE) * (c0 + jl)] : C{0, 0};
-    }
+#pragma unroll
+    for (int u = 0; u < NA; ++u)
+      Al[threadIdx.x + 256 * u] = pa[u];
+#pragma unroll
+    for (int u = 0; u < NT; ++u)
+      if (threadIdx.x + 256 * u < JV * WT)
+        Tl[threadIdx.x + 256 * u] = pt[u];
     __syncthreads();
+    if (c1 < n)
+      prefetch(c1);
     if (!active)
       continue;
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
       const int j0 = O[r] > c0 ? O[r] : c0;
       const int j1 = O[r + 1] < c1 ? O[r + 1] : c1;
-      int jj = j0;
-      for (; jj + 4 <= j1; jj += 4) {
-        C av[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          av[u] = As[static_cast<size_t>(g.D) * (jj + u)];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const C* tl = Tl + (jj + u - c0) * WT + wave * TW;
-#pragma unroll
-          for (int tt = 0; tt < TW; ++tt)
-            if (tt < tw_ok)
-              cfmac(e[tt][r], av[u], tl[tt]);
-        }
-      }
-      for (; jj < j1; ++jj) {
-        const C av = As[static_cast<size_t>(g.D) * jj];
+#pragma unroll 4
+      for (int jj = j0; jj < j1; ++jj) {
+        const C av = Al[(jj - c0) * 64 + lane];
         const C* tl = Tl + (jj - c0) * WT + wave * TW;
 #pragma unroll
         for (int tt = 0; tt < TW; ++tt)

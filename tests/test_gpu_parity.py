@@ -255,6 +255,7 @@ def test_mfma_rank_one_matches_oracle(oracle_lib, name, channels):
     ref = oracle_g4(oracle_lib, g, ms)
     mf = g4acc.TpAccumulator(g, precision="fp32", max_walkers=9, max_vertices=wl.n_max, tuned=False)
     vec = g4acc.TpAccumulator(g, precision="fp32", max_walkers=9, max_vertices=wl.n_max, tuned=False)
+    mf.set_tuning(g4_mfma=1)
     vec.set_tuning(g4_mfma=0)
     for acc in (mf, vec):
         acc.resetAccumulation()
