@@ -45,6 +45,9 @@ struct g4acc_handle {
   size_t bytes = 0;
   int *d_kadd = nullptr, *d_ksub = nullptr, *d_kex = nullptr, *d_wex = nullptr;
   int* d_tperm = nullptr;          // G4 tile schedule 2 (diagonal strips)
+  // Square-cluster site labels (x + L y) and the caller momentum of each (kx, ky); L = 0: none.
+  int grid_L = 0;
+  int *d_site_map = nullptr, *d_kmap = nullptr;
   void* d_dft = nullptr;           // Cplx<Real>[Nc*Nc]
   double2* d_bphase = nullptr;     // [nb*Nc] or null
   double2* d_G0 = nullptr;
@@ -167,6 +170,62 @@ void upload_dft(std::vector<unsigned char>& buf, const double* dft, int nc) {
     p[i] = C{static_cast<Real>(dft[2 * i]), static_cast<Real>(dft[2 * i + 1])};
 }
 
+// Finds whether the cluster DFT dft[k + nc r] = exp(i k.r) is that of an L x L torus: site and
+// momentum coordinates (x, y), (kx, ky) in Z_L with dft = exp(2 pi i (kx x + ky y) / L). Then
+// xy[r] = x + L y and kxy_to_k[kx + L ky] = k. Only L = 2, 4, 6, 8 are used (the separable right
+// site DFT is instantiated for them). Returns L, or 0.
+int detect_grid(int nc, const double* dft, std::vector<int>& xy, std::vector<int>& kxy_to_k) {
+  const int L = static_cast<int>(std::lround(std::sqrt(static_cast<double>(nc))));
+  if (L * L != nc || (L != 2 && L != 4 && L != 6 && L != 8))
+    return 0;
+  const double two_pi = 6.283185307179586476925;
+  std::vector<int> P(static_cast<size_t>(nc) * nc);  // P[k + nc r]: exponent in Z_L
+  for (int i = 0; i < nc * nc; ++i) {
+    const double re = dft[2 * i], im = dft[2 * i + 1];
+    int m = static_cast<int>(std::lround(std::atan2(im, re) * L / two_pi));
+    m = ((m % L) + L) % L;
+    if (std::abs(re - std::cos(two_pi * m / L)) > 1e-9 || std::abs(im - std::sin(two_pi * m / L)) > 1e-9)
+      return 0;
+    P[i] = m;
+  }
+  std::vector<int> kx(nc), ky(nc), seen(nc);
+  for (int r1 = 0; r1 < nc; ++r1)
+    for (int r2 = 0; r2 < nc; ++r2) {
+      // Candidate unit sites (1,0) and (0,1): k -> (P[k][r1], P[k][r2]) must be a bijection.
+      std::fill(seen.begin(), seen.end(), 0);
+      bool bij = true;
+      for (int k = 0; k < nc && bij; ++k) {
+        kx[k] = P[k + nc * r1];
+        ky[k] = P[k + nc * r2];
+        bij = !seen[kx[k] + L * ky[k]]++;
+      }
+      if (!bij)
+        continue;
+      // Every site then has coordinates (x, y) with P[k][r] = kx x + ky y for all k.
+      std::vector<int> xy_try(nc, -1), used(nc, 0);
+      bool all = true;
+      for (int r = 0; r < nc && all; ++r) {
+        for (int c = 0; c < nc && xy_try[r] < 0; ++c) {
+          const int x = c % L, y = c / L;
+          bool fits = true;
+          for (int k = 0; k < nc && fits; ++k)
+            fits = P[k + nc * r] == (kx[k] * x + ky[k] * y) % L;
+          if (fits)
+            xy_try[r] = c;
+        }
+        all = xy_try[r] >= 0 && !used[xy_try[r]]++;
+      }
+      if (!all)
+        continue;
+      xy = xy_try;
+      kxy_to_k.assign(nc, 0);
+      for (int k = 0; k < nc; ++k)
+        kxy_to_k[kx[k] + L * ky[k]] = k;
+      return L;
+    }
+  return 0;
+}
+
 // Resolves the walkers of one chunk into slot descriptors; host inputs are packed into the
 // pinned staging buffer and copied once. Returns the number of slots.
 int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int memory,
@@ -263,12 +322,14 @@ MtoGArgs<Real> mtog_args(g4acc_handle* h, int n_slots) {
   a.dft = static_cast<const Cplx<Real>*>(h->d_dft);
   a.bphase = h->d_bphase;
   a.G0 = h->d_G0;
+  a.grid_L = h->grid_L;
+  a.kmap = h->d_kmap;
   return a;
 }
 
 template <typename Real>
 void run_mtog(g4acc_handle* h, int n_slots, hipStream_t st) {
-  launch_sort(h->d_desc, h->g, n_slots, h->max_vertices, h->d_perm, h->d_off, h->d_err, st);
+  launch_sort(h->d_desc, h->g, n_slots, h->max_vertices, h->d_site_map, h->d_perm, h->d_off, h->d_err, st);
   launch_mtog<Real>(mtog_args<Real>(h, n_slots), st);
 }
 
@@ -559,13 +620,31 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
                      "hipHostMalloc")))
     return bail(rc);
 
-  // Constant tables.
+  // Constant tables. On a square cluster the sites are relabelled x + L y (the sort orders the
+  // vertices by that label and the DFT columns follow it), so the right site DFT can run as two
+  // L-point passes (G4ACC_DENSE_DFT=1 keeps the caller's labels and the dense DFT).
   const std::vector<int> tperm = diagonal_tile_order(g.nw, 2);
+  std::vector<int> xy, kxy_to_k;
+  if (getenv("G4ACC_DENSE_DFT") == nullptr)
+    h->grid_L = detect_grid(g.nc, gm->dft, xy, kxy_to_k);
+  std::vector<double> dft_lab(gm->dft, gm->dft + 2 * static_cast<size_t>(nc2));
+  if (h->grid_L) {
+    for (int r = 0; r < g.nc; ++r)
+      for (int k = 0; k < g.nc; ++k) {
+        dft_lab[2 * (k + static_cast<size_t>(g.nc) * xy[r])] = gm->dft[2 * (k + static_cast<size_t>(g.nc) * r)];
+        dft_lab[2 * (k + static_cast<size_t>(g.nc) * xy[r]) + 1] =
+            gm->dft[2 * (k + static_cast<size_t>(g.nc) * r) + 1];
+      }
+    if ((rc = h->alloc(&h->d_site_map, g.nc, "site map")) || (rc = h->alloc(&h->d_kmap, g.nc, "k map")) ||
+        (rc = h->check(hipMemcpy(h->d_site_map, xy.data(), sizeof(int) * g.nc, hipMemcpyHostToDevice), "copy")) ||
+        (rc = h->check(hipMemcpy(h->d_kmap, kxy_to_k.data(), sizeof(int) * g.nc, hipMemcpyHostToDevice), "copy")))
+      return bail(rc);
+  }
   std::vector<unsigned char> dftbuf;
   if (h->precision == G4ACC_FP64)
-    upload_dft<double>(dftbuf, gm->dft, g.nc);
+    upload_dft<double>(dftbuf, dft_lab.data(), g.nc);
   else
-    upload_dft<float>(dftbuf, gm->dft, g.nc);
+    upload_dft<float>(dftbuf, dft_lab.data(), g.nc);
   if ((rc = h->check(hipMemcpy(h->d_kadd, gm->k_add, sizeof(int) * nc2, hipMemcpyHostToDevice), "copy")) ||
       (rc = h->check(hipMemcpy(h->d_ksub, gm->k_sub, sizeof(int) * nc2, hipMemcpyHostToDevice), "copy")) ||
       (rc = h->check(hipMemcpy(h->d_kex, gm->k_ex, sizeof(int) * g.nq, hipMemcpyHostToDevice), "copy")) ||

@@ -24,10 +24,15 @@ struct MtoGArgs {
   const Cplx<Real>* dft;  // [Nc][Nc] (k fastest)
   const double2* bphase;  // [Nc][nb] or nullptr
   const double2* G0;      // (b1, b2, s, k, wE)
+  // Square L x L clusters (L = 2, 4, 6, 8): sites are relabelled x + L y and the right site DFT
+  // runs as two L-point passes; kmap[kx + L ky] is the caller's momentum index. L = 0: dense DFT.
+  int grid_L;
+  const int* kmap;
 };
 
-void launch_sort(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, int* perm, int* off,
-                 int* err_flag, hipStream_t st);
+// site_map (or nullptr): the internal label of each caller site (sites sorted by it).
+void launch_sort(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, const int* site_map,
+                 int* perm, int* off, int* err_flag, hipStream_t st);
 
 template <typename Real>
 void launch_mtog(const MtoGArgs<Real>& a, hipStream_t st);

@@ -59,12 +59,83 @@ __global__ void __launch_bounds__(256) mtog_left_fused(const SlotDesc* __restric
   }
 }
 
+// Side of the square cluster whose site DFT factorises into two L-point passes (0: none).
+template <int NC>
+constexpr int grid_side() {
+  return NC == 4 ? 2 : NC == 16 ? 4 : NC == 36 ? 6 : NC == 64 ? 8 : 0;
+}
+
+// cos(2 pi m / L) and sin(2 pi m / L) for the sides above (0 <= m < L).
+__host__ __device__ constexpr double unit_cos(int L, int m) {
+  return L == 6 ? (m == 0 ? 1.0 : m == 3 ? -1.0 : (m == 1 || m == 5) ? 0.5 : -0.5)
+                : (m == 0 ? 1.0 : m == 4 ? -1.0 : (m == 2 || m == 6) ? 0.0
+                                                : (m == 1 || m == 7) ? 0.70710678118654752440
+                                                                     : -0.70710678118654752440);
+}
+__host__ __device__ constexpr double unit_sin(int L, int m) {
+  return L == 6 ? ((m == 0 || m == 3) ? 0.0 : (m == 1 || m == 2) ? 0.86602540378443864676
+                                                                 : -0.86602540378443864676)
+                : ((m == 0 || m == 4) ? 0.0 : m == 2 ? 1.0 : m == 6 ? -1.0
+                                         : (m == 1 || m == 3) ? 0.70710678118654752440
+                                                              : -0.70710678118654752440);
+}
+
+// v * exp(-2 pi i m / L). Quarter turns are exact swaps and negations; the loops that call this
+// are fully unrolled, so m and the branch are compile-time constants.
+template <int L, typename Real>
+__device__ __forceinline__ Cplx<Real> rot(Cplx<Real> v, int m) {
+  m %= L;
+  if ((4 * m) % L == 0) {
+    const int q = (4 * m) / L;
+    return q == 0 ? v : q == 1 ? Cplx<Real>{v.y, -v.x} : q == 2 ? Cplx<Real>{-v.x, -v.y} : Cplx<Real>{-v.y, v.x};
+  }
+  const Real c = static_cast<Real>(unit_cos(L, m)), sn = static_cast<Real>(unit_sin(L, m));
+  return Cplx<Real>{v.x * c + v.y * sn, v.y * c - v.x * sn};
+}
+
+// In place: v[kx + L ky] <- sum_{x,y} v[x + L y] exp(-2 pi i (kx x + ky y) / L), as two passes of
+// L-point sums (2 L^3 rotations instead of L^4 products for the dense site DFT).
+template <int L, typename Real>
+__device__ __forceinline__ void site_dft_2d(Cplx<Real> (&v)[L * L]) {
+  using C = Cplx<Real>;
+#pragma unroll
+  for (int x = 0; x < L; ++x) {
+    C col[L];
+#pragma unroll
+    for (int y = 0; y < L; ++y)
+      col[y] = v[x + L * y];
+#pragma unroll
+    for (int ky = 0; ky < L; ++ky) {
+      C acc = col[0];
+#pragma unroll
+      for (int y = 1; y < L; ++y)
+        acc = cadd(acc, rot<L, Real>(col[y], ky * y));
+      v[x + L * ky] = acc;
+    }
+  }
+#pragma unroll
+  for (int ky = 0; ky < L; ++ky) {
+    C row[L];
+#pragma unroll
+    for (int x = 0; x < L; ++x)
+      row[x] = v[x + L * ky];
+#pragma unroll
+    for (int kx = 0; kx < L; ++kx) {
+      C acc = row[0];
+#pragma unroll
+      for (int x = 1; x < L; ++x)
+        acc = cadd(acc, rot<L, Real>(row[x], kx * x));
+      v[kx + L * ky] = acc;
+    }
+  }
+}
+
 // Per chunk of JV sorted vertices the workgroup stages in LDS the A rows of its 64 p1 (all four
 // waves read the same rows) and the phases of its 4 TW frequencies; both are loaded
 // cooperatively, eight A elements per thread in flight, and the next chunk is loaded into
 // registers while the current one is consumed. The cluster DFT matrix and the G0 slice of the
 // workgroup's frequencies are staged once.
-template <typename Real, int NC, int TW>
+template <typename Real, int NC, int TW, bool SEP>
 __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restrict__ desc, Geo g,
                                                         int n_max, int slot0,
                                                         const int* __restrict__ off,
@@ -73,8 +144,10 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
                                                         const Cplx<Real>* __restrict__ dft,
                                                         const double2* __restrict__ bphase,
                                                         const double2* __restrict__ G0,
+                                                        const int* __restrict__ kmap,
                                                         Cplx<Real>* __restrict__ G) {
   using C = Cplx<Real>;
+  constexpr int L = SEP ? grid_side<NC>() : 0;
   constexpr int WT = 4 * TW;                  // frequencies per workgroup
   constexpr int JV = 16;                      // vertices per staged chunk
   constexpr int NA = JV * 64 / 256;           // A elements loaded per thread and chunk
@@ -83,6 +156,7 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
   __shared__ C Tl[JV * WT];
   __shared__ C Dl[NC * NC];
   __shared__ double2 G0l[NC * WT];
+  __shared__ int Kl[NC];                      // caller momentum of each (kx, ky) (separable DFT)
   const int slot = slot0 + blockIdx.z;
   const int s = slot & 1;
   const int lane = threadIdx.x & 63;
@@ -99,6 +173,8 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
 
   for (int i = threadIdx.x; i < NC * NC; i += 256)
     Dl[i] = dft[i];
+  if (threadIdx.x < NC)
+    Kl[threadIdx.x] = kmap ? kmap[threadIdx.x] : threadIdx.x;
   for (int i = threadIdx.x; i < NC * WT; i += 256) {
     const int k = i % NC, wl = i / NC, w = w2wg + wl;
     G0l[i] = w < g.nwE ? G0[s + 2 * (k + static_cast<size_t>(NC) * w)] : double2{0, 0};
@@ -167,6 +243,38 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
   const double2 ga = G0[s + 2 * (k1 + static_cast<size_t>(NC) * w1)];
   const double inv_nc = 1.0 / NC;
   C* Gs = G + static_cast<size_t>(slot) * g.D * g.D + p1;
+  if constexpr (L > 0) {
+    {
+      // Square cluster: sites are labelled x + L y, so the DFT over them is two L-point passes;
+      // output kx + L ky is the caller's momentum Kl[kx + L ky].
+#pragma unroll
+      for (int tt = 0; tt < TW; ++tt) {
+        const int w2 = w2b + tt;
+        if (w2 >= g.nwE)
+          break;
+        site_dft_2d<L, Real>(e[tt]);
+#pragma unroll
+        for (int kk = 0; kk < NC; ++kk) {
+          const int k2 = Kl[kk];
+          double2 m = to_d(e[tt][kk]);
+          m.x *= inv_nc;
+          m.y *= inv_nc;
+          if (bphase)
+            m = cmulc(cmul(m, bphase[k1]), bphase[k2]);
+          double2 v = cmul(cmul(ga, m), G0l[k2 + NC * (w2 - w2wg)]);
+          v.x = -v.x;
+          v.y = -v.y;
+          const int p2 = k2 + NC * w2;
+          if (p1 == p2) {
+            v.x = fma(ga.x, g.beta, v.x);
+            v.y = fma(ga.y, g.beta, v.y);
+          }
+          Gs[static_cast<size_t>(g.D) * p2] = from_d<C>(v);
+        }
+      }
+      return;
+    }
+  }
 #pragma unroll
   for (int tt = 0; tt < TW; ++tt) {
     const int w2 = w2b + tt;
@@ -200,11 +308,19 @@ namespace {
 
 template <typename Real, int NC, int TW>
 void launch_fused_nc(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st) {
+  // The separable site DFT is instantiated for the 2x2 and 4x4 clusters; for 6x6 and 8x8 the
+  // fully unrolled passes cost more registers than the dense DFT saves.
+  constexpr bool kSep = NC == 4 || NC == 16;
   const Geo& g = a.g;
   hipLaunchKernelGGL((mtog_left_fused<Real, NC>), dim3((g.nwE * a.n_max + 255) / 256, 1, ns), dim3(256), 0,
                      st, a.desc, g, a.n_max, s0, a.perm, a.off, a.T, a.dft, a.A);
-  hipLaunchKernelGGL((mtog_right_fused<Real, NC, TW>), dim3((g.D + 63) / 64, (g.nwE + 4 * TW - 1) / (4 * TW), ns),
-                     dim3(256), 0, st, a.desc, g, a.n_max, s0, a.off, a.T, a.A, a.dft, a.bphase, a.G0, a.G);
+  const dim3 grid((g.D + 63) / 64, (g.nwE + 4 * TW - 1) / (4 * TW), ns);
+  if (kSep && a.grid_L == grid_side<NC>() && a.kmap)
+    hipLaunchKernelGGL((mtog_right_fused<Real, NC, TW, kSep>), grid, dim3(256), 0, st, a.desc, g, a.n_max, s0,
+                       a.off, a.T, a.A, a.dft, a.bphase, a.G0, a.kmap, a.G);
+  else
+    hipLaunchKernelGGL((mtog_right_fused<Real, NC, TW, false>), grid, dim3(256), 0, st, a.desc, g, a.n_max, s0,
+                       a.off, a.T, a.A, a.dft, a.bphase, a.G0, nullptr, a.G);
 }
 
 }  // namespace

@@ -32,7 +32,8 @@ namespace g4acc {
 // details::Triple (tp/ndft/triple.hpp:23-30). perm[slot][rank] = vertex index, off[slot][o] =
 // first rank of orbital o (n_orb + 1 entries). Invalid site/band indices raise *err_flag.
 __global__ void __launch_bounds__(256) sort_kernel(const SlotDesc* __restrict__ desc, Geo g,
-                                                   int n_max, int* __restrict__ perm,
+                                                   int n_max, const int* __restrict__ site_map,
+                                                   int* __restrict__ perm,
                                                    int* __restrict__ off, int* __restrict__ err_flag) {
   extern __shared__ int lds[];
   int* orb = lds;               // n_max
@@ -47,11 +48,11 @@ __global__ void __launch_bounds__(256) sort_kernel(const SlotDesc* __restrict__ 
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int b = d.band[i], r = d.site[i];
-    int o = b + g.nb * r;
-    if (b < 0 || b >= g.nb || r < 0 || r >= g.nc) {
+    int o = 0;
+    if (b < 0 || b >= g.nb || r < 0 || r >= g.nc)
       atomicOr(err_flag, 1);
-      o = 0;
-    }
+    else
+      o = b + g.nb * (site_map ? site_map[r] : r);
     orb[i] = o;
     atomicAdd(&cnt[o + 1], 1);
   }
@@ -240,10 +241,11 @@ __global__ void __launch_bounds__(256) dft_right_G_kernel(Geo g, int slot0,
 // ---------------------------------------------------------------------------------------------
 // Host launchers.
 
-void launch_sort(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, int* perm, int* off,
-                 int* err_flag, hipStream_t st) {
+void launch_sort(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, const int* site_map,
+                 int* perm, int* off, int* err_flag, hipStream_t st) {
   const size_t lds = sizeof(int) * (n_max + g.n_orb + 1);
-  hipLaunchKernelGGL(sort_kernel, dim3(n_slots), dim3(256), lds, st, desc, g, n_max, perm, off, err_flag);
+  hipLaunchKernelGGL(sort_kernel, dim3(n_slots), dim3(256), lds, st, desc, g, n_max, site_map, perm, off,
+                     err_flag);
 }
 
 template <typename Real>

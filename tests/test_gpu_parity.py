@@ -283,6 +283,22 @@ def test_fused_equals_unfused_mtog(name, precision, monkeypatch):
         assert err < (1e-14 if precision == "fp64" else 1e-6)
 
 
+@pytest.mark.parametrize("name", ["1band_2x2", "1band_4x4_4q", "1band_6x6", "3band_2x2"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_separable_site_dft_equals_dense(name, precision, monkeypatch):
+    """On square clusters the sites are relabelled x + L y and the right site DFT runs as two
+    L-point passes; G equals the dense-DFT path with the caller's labels."""
+    wl = case(name)
+    sep = g4acc.TpAccumulator(wl.geometry, precision=precision, max_vertices=wl.n_max)
+    monkeypatch.setenv("G4ACC_DENSE_DFT", "1")
+    dense = g4acc.TpAccumulator(wl.geometry, precision=precision, max_vertices=wl.n_max)
+    for meas in wl.draw(2):
+        a, b = sep.compute_G(meas), dense.compute_G(meas)
+        err = linf(b, a)
+        print(f"{name} {precision} separable vs dense site DFT G l_inf={err:.3e}", flush=True)
+        assert err < (1e-14 if precision == "fp64" else 1e-6)
+
+
 @pytest.mark.parametrize("name", ["1band_4x4_4q", "3band_2x2"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_graph_replays_equal_eager(name, precision, monkeypatch):
