@@ -57,6 +57,7 @@ struct g4acc_handle {
   void* d_UV = nullptr;            // rank-one factors of the tiled single-band G4 kernel
   bool fused_mtog = false;         // register-fused single-band M -> G (no E workspace)
   bool use_tiled = false;
+  bool use_band = false;           // LDS-tiled multiband G4 kernel (G4ACC_GENERIC_G4=1 disables)
   int tile_order = -1;             // G4 tile schedule (G4ACC_TILE_ORDER); -1 = by batch size
   SlotDesc* d_desc = nullptr;
   SlotDesc* h_desc = nullptr;      // pinned
@@ -366,6 +367,27 @@ void run_g4(g4acc_handle* h, int n_walkers) {
     if (launch_g4_tiled<Real>(a, h->stream))
       return;
   }
+  else if (h->g.nb > 1 && h->use_band && n_walkers >= 4) {
+    // The staged blocks pay off over several walkers; a single walker's entries are cheaper to
+    // gather directly (generic kernel).
+    TiledArgs<Real> a{};
+    a.g = h->g;
+    a.G = static_cast<const Cplx<Real>*>(h->d_G);
+    a.desc = h->d_desc;
+    a.n_walkers = n_walkers;
+    a.n_ch = static_cast<int>(h->channels.size());
+    for (int c = 0; c < a.n_ch; ++c) {
+      a.ch[c] = h->channels[c];
+      a.g4[c] = h->d_g4[c];
+    }
+    a.k_add = h->d_kadd;
+    a.k_sub = h->d_ksub;
+    a.k_ex = h->d_kex;
+    a.w_ex = h->d_wex;
+    a.sum_mode = h->sum_mode;
+    if (launch_g4_band<Real>(a, h->stream))
+      return;
+  }
   for (size_t c = 0; c < h->channels.size(); ++c) {
     G4Args<Real> a;
     a.g = h->g;
@@ -598,6 +620,9 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
       h->use_graph = atoi(gr) != 0;
     h->use_tiled = g.nb == 1 && g.nc >= 8 && tiled_pairs_per_thread(g.nc) > 0 &&
                    lds <= static_cast<size_t>(max_lds) && getenv("G4ACC_GENERIC_G4") == nullptr;
+    const size_t R = static_cast<size_t>(g.nb) * g.nc;
+    const size_t band_lds = cb * (6 * R * (R + 1) + 4 * static_cast<size_t>(g.nb) * g.nb * g.nc);
+    h->use_band = g.nb > 1 && band_lds <= static_cast<size_t>(max_lds) && getenv("G4ACC_GENERIC_G4") == nullptr;
     if (h->use_tiled &&
         (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_UV),
                        cb * W2 / 2 * 4 * g.nq * g.nv * g.nc * g.nw, "UV")))

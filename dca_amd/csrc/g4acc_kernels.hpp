@@ -93,6 +93,10 @@ bool launch_g4_tiled(TiledArgs<Real> a, hipStream_t st);
 // false if it does not cover the geometry or channel set. The caller has run uv_kernel and set
 // a.nqg for momentum groups of qg.
 bool launch_g4_mfma(const TiledArgs<float>& a, int qg, hipStream_t st);
+// Multiband LDS-tiled G4 update (g4_band.hip); false if the band count, cluster or channel set
+// is not covered (the generic kernel then runs).
+template <typename Real>
+bool launch_g4_band(TiledArgs<Real> a, hipStream_t st);
 int tiled_pairs_per_thread(int nc);
 // Tile schedule 2: (w1 | w2 << 16) for all nw^2 frequency pairs, in dispatch order.
 std::vector<int> diagonal_tile_order(int nw, int strip);

@@ -191,10 +191,10 @@ def test_sum_to_and_errors(oracle_lib):
         b.accumulate(bad)
 
 
-@pytest.mark.parametrize("name", ["1band_4x4_4q", "1band_6x6", "1band_8site"])
+@pytest.mark.parametrize("name", ["1band_4x4_4q", "1band_6x6", "1band_8site", "3band_2x2", "2band_8site"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_tiled_equals_generic_kernel(name, precision, monkeypatch):
-    """The LDS-tiled single-band G4 kernel and the generic per-entry kernel agree."""
+    """The LDS-tiled G4 kernels (single band, multiband) and the generic per-entry kernel agree."""
     wl = case(name, channels=[geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE,
                               geo.PARTICLE_PARTICLE_UP_DOWN])
     g = wl.geometry

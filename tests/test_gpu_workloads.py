@@ -59,7 +59,7 @@ def test_workload_free_closed_form(cfg, precision, tol):
         del ref
 
 
-@pytest.mark.parametrize("cfg,count", [(2, 6), (3, 3), (4, 3), (5, 2)])
+@pytest.mark.parametrize("cfg,count", [(2, 6), (3, 3), (4, 4), (5, 2)])
 def test_workload_batched_and_mixed(cfg, count):
     wl = synth.workload(cfg)
     ms = wl.draw(count)
