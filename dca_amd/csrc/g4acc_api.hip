@@ -66,6 +66,19 @@ struct g4acc_handle {
   double* h_stage_f = nullptr;     // pinned
   int* h_stage_i = nullptr;
   size_t stage_f_cap = 0, stage_i_cap = 0;
+  // Queue of g4acc_accumulate_queued: owned copies of host inputs; device inputs are copied
+  // into d_qm (M), d_qtau, d_qsb (site, band) slot by slot.
+  struct Queued {
+    int n[2] = {0, 0};
+    double sign = 1;
+    std::vector<double> m[2], tau[2];
+    std::vector<int> site[2], band[2];
+  };
+  std::vector<Queued> queue;
+  int queue_kind = G4ACC_HOST;
+  double* d_qm = nullptr;
+  double* d_qtau = nullptr;
+  int* d_qsb = nullptr;
   hipEvent_t staged = nullptr;     // staging buffers free once this event completes
   bool staged_pending = false;
   // Captured launch sequences, one per batch size (G4ACC_GRAPH=0 disables them).
@@ -725,6 +738,7 @@ int g4acc_reset(g4acc_handle* h) {
   if (!h)
     return G4ACC_INVALID_ARG;
   G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  h->queue.clear();
   for (auto* p : h->d_g4)
     launch_zero16(p, h->g4_entries, h->stream);
   h->initialized = true;
@@ -763,10 +777,123 @@ int g4acc_accumulate(g4acc_handle* h, const g4acc_walker* walker, int memory) {
   return g4acc_accumulate_batch(h, walker, 1, memory);
 }
 
+int g4acc_flush(g4acc_handle* h) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  if (h->queue.empty())
+    return G4ACC_OK;
+  const size_t NM = h->max_vertices, NM2 = NM * NM;
+  const int kind = h->queue_kind;
+  std::vector<g4acc_walker> ws(h->queue.size());
+  for (size_t i = 0; i < ws.size(); ++i) {
+    const auto& q = h->queue[i];
+    g4acc_walker& w = ws[i];
+    w.sign = q.sign;
+    for (int s = 0; s < 2; ++s) {
+      const size_t sl = 2 * i + s;
+      w.n[s] = q.n[s];
+      w.ld[s] = q.n[s] > 0 ? q.n[s] : 1;
+      if (q.n[s] == 0) {
+        w.m[s] = nullptr;
+        w.tau[s] = nullptr;
+        w.site[s] = w.band[s] = nullptr;
+        continue;
+      }
+      w.m[s] = kind == G4ACC_HOST ? q.m[s].data() : h->d_qm + sl * NM2;
+      if (kind == G4ACC_DEVICE) {
+        w.tau[s] = h->d_qtau + sl * NM;
+        w.site[s] = h->d_qsb + 2 * sl * NM;
+        w.band[s] = h->d_qsb + (2 * sl + 1) * NM;
+      }
+      else {
+        w.tau[s] = q.tau[s].data();
+        w.site[s] = q.site[s].data();
+        w.band[s] = q.band[s].data();
+      }
+    }
+  }
+  // The batch reads host copies before it returns and device copies in stream order, so the
+  // queue slots are free for the next measurements.
+  const int rc = g4acc_accumulate_batch(h, ws.data(), static_cast<int>(ws.size()), kind);
+  h->queue.clear();
+  return rc;
+}
+
+int g4acc_accumulate_queued(g4acc_handle* h, const g4acc_walker* walker, int memory) {
+  if (!h || !walker)
+    return G4ACC_INVALID_ARG;
+  if (!h->initialized)
+    return h->fail(G4ACC_NOT_INITIALIZED, "accumulate called before resetAccumulation");
+  if (memory != G4ACC_HOST && memory != G4ACC_DEVICE_M && memory != G4ACC_DEVICE)
+    return h->fail(G4ACC_INVALID_ARG, "invalid memory kind");
+  if (!h->g4_present)
+    return G4ACC_OK;
+  G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  const g4acc_walker& w = *walker;
+  G4_TRY(check_walker(h, w));
+  if (w.n[0] + w.n[1] == 0)
+    return G4ACC_OK;  // empty configuration: no-op (tp_accumulator_cpu.hpp:238-239)
+  if (!h->queue.empty() && h->queue_kind != memory)
+    G4_TRY(g4acc_flush(h));
+  const size_t NM = h->max_vertices, NM2 = NM * NM, W2 = 2 * static_cast<size_t>(h->max_walkers);
+  if (memory != G4ACC_HOST && !h->d_qm)
+    G4_TRY(h->alloc(&h->d_qm, W2 * NM2, "queue M"));
+  if (memory == G4ACC_DEVICE && !h->d_qtau) {
+    G4_TRY(h->alloc(&h->d_qtau, W2 * NM, "queue tau"));
+    G4_TRY(h->alloc(&h->d_qsb, 2 * W2 * NM, "queue site/band"));
+  }
+  const size_t slot = h->queue.size();
+  g4acc_handle::Queued q;
+  q.sign = w.sign;
+  for (int s = 0; s < 2; ++s) {
+    const int n = w.n[s];
+    q.n[s] = n;
+    if (n == 0)
+      continue;
+    const size_t sl = 2 * slot + s;
+    if (memory == G4ACC_HOST) {
+      q.m[s].resize(static_cast<size_t>(n) * n);
+      for (int j = 0; j < n; ++j)
+        std::memcpy(q.m[s].data() + static_cast<size_t>(n) * j, w.m[s] + static_cast<size_t>(w.ld[s]) * j,
+                    sizeof(double) * n);
+    }
+    else
+      G4_TRY(h->check(hipMemcpy2DAsync(h->d_qm + sl * NM2, sizeof(double) * n, w.m[s],
+                                       sizeof(double) * w.ld[s], sizeof(double) * n, n,
+                                       hipMemcpyDeviceToDevice, h->stream), "queue M copy"));
+    if (memory == G4ACC_DEVICE) {
+      G4_TRY(h->check(hipMemcpyAsync(h->d_qtau + sl * NM, w.tau[s], sizeof(double) * n,
+                                     hipMemcpyDeviceToDevice, h->stream), "queue tau copy"));
+      G4_TRY(h->check(hipMemcpyAsync(h->d_qsb + 2 * sl * NM, w.site[s], sizeof(int) * n,
+                                     hipMemcpyDeviceToDevice, h->stream), "queue site copy"));
+      G4_TRY(h->check(hipMemcpyAsync(h->d_qsb + (2 * sl + 1) * NM, w.band[s], sizeof(int) * n,
+                                     hipMemcpyDeviceToDevice, h->stream), "queue band copy"));
+    }
+    else {
+      q.tau[s].assign(w.tau[s], w.tau[s] + n);
+      q.site[s].assign(w.site[s], w.site[s] + n);
+      q.band[s].assign(w.band[s], w.band[s] + n);
+    }
+  }
+  h->queue.push_back(std::move(q));
+  h->queue_kind = memory;
+  if (static_cast<int>(h->queue.size()) == h->max_walkers)
+    return g4acc_flush(h);
+  return G4ACC_OK;
+}
+
+int g4acc_sync_copy(g4acc_handle* h) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  return h->check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+}
+
 int g4acc_sync(g4acc_handle* h) {
   if (!h)
     return G4ACC_INVALID_ARG;
   G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  G4_TRY(g4acc_flush(h));
   G4_TRY(quiesce(h));
   int flag = 0;
   G4_TRY(h->check(hipMemcpy(&flag, h->d_err, sizeof(int), hipMemcpyDeviceToHost), "read error flag"));
@@ -810,6 +937,7 @@ int g4acc_sum_to(g4acc_handle* src, g4acc_handle* dst) {
   if (!src->g4_present || !dst->g4_present)
     return src->fail(G4ACC_EMPTY, "There is no G4 stored in this class.");
   G4_TRY(src->check(hipSetDevice(src->device), "hipSetDevice"));
+  G4_TRY(g4acc_flush(src));
   G4_TRY(src->check(hipStreamSynchronize(src->stream), "hipStreamSynchronize"));
   for (size_t c = 0; c < src->channels.size(); ++c)
     launch_g4_add(dst->d_g4[c], src->d_g4[c], src->g4_entries, src->sum_mode, dst->stream);

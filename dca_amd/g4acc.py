@@ -31,7 +31,8 @@ EXPORTED = [
     "g4acc_create", "g4acc_destroy", "g4acc_reset", "g4acc_accumulate", "g4acc_accumulate_batch",
     "g4acc_finalize", "g4acc_sync", "g4acc_get_g4", "g4acc_sum_to", "g4acc_g4_size", "g4acc_bytes",
     "g4acc_stream", "g4acc_last_error", "g4acc_compute_G", "g4acc_g_size", "g4acc_accumulate_G",
-    "g4acc_set_timing", "g4acc_get_timing", "g4acc_set_tuning",
+    "g4acc_set_timing", "g4acc_get_timing", "g4acc_set_tuning", "g4acc_accumulate_queued",
+    "g4acc_flush", "g4acc_sync_copy",
 ]
 
 
@@ -95,6 +96,9 @@ def load_library(path: str = LIB_PATH):
         "g4acc_accumulate_G": (i32, [vp, vp, ctypes.c_double]),
         "g4acc_set_timing": (i32, [vp, i32]),
         "g4acc_set_tuning": (i32, [vp, ctypes.c_char_p, i32]),
+        "g4acc_accumulate_queued": (i32, [vp, ctypes.POINTER(Walker_c), i32]),
+        "g4acc_flush": (i32, [vp]),
+        "g4acc_sync_copy": (i32, [vp]),
         "g4acc_get_timing": (i32, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_longlong)]),
     }
@@ -237,6 +241,22 @@ class TpAccumulator:
         arr = (Walker_c * len(measurements))(*[walker_struct(m, keep) for m in measurements])
         self._check(self.lib.g4acc_accumulate_batch(self._h, arr, len(measurements), HOST))
         self._inflight.append(keep)
+
+    def accumulate_queued(self, meas, memory: int = HOST, walker: Walker_c | None = None) -> None:
+        """accumulate(M_pair, configs, sign) through the handle's queue: the measurement is copied
+        and runs in a batch of max_walkers (or at flush / sync / get_G4 / sumTo). Host
+        measurements are given as `meas`; device ones as a ready `walker` (device_walker_struct)."""
+        if walker is None:
+            keep = _Keep()
+            walker = walker_struct(meas, keep)
+        self._check(self.lib.g4acc_accumulate_queued(self._h, ctypes.byref(walker), memory))
+
+    def flush(self) -> None:
+        self._check(self.lib.g4acc_flush(self._h))
+
+    def sync_copy(self) -> None:
+        """synchronizeCopy for queued measurements: their inputs have been copied."""
+        self._check(self.lib.g4acc_sync_copy(self._h))
 
     def accumulate_device(self, walkers: Sequence[Walker_c]) -> None:
         """Batch of walkers whose arrays are device memory (see device_walker_struct)."""

@@ -119,9 +119,22 @@ int g4acc_accumulate(g4acc_handle* h, const g4acc_walker* walker, int memory);
 /* Many walkers' measurements in one batched launch sequence; equivalent to calling
  * g4acc_accumulate for each walker. n_walkers may exceed max_walkers (processed in chunks). */
 int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_walkers, int memory);
-/* finalize() (tp_accumulator_gpu.hpp:471-484): waits for all queued work. */
+/* accumulate(M_pair, configs, sign) with batching inside the handle, for callers that hand over
+ * one measurement at a time (the reference launches each one on its own,
+ * tp_accumulator_gpu.hpp:321-355). The measurement is copied into the handle's queue: host
+ * arrays before the call returns, device arrays by a device copy on the handle's stream. The
+ * queue runs as one batch when it holds max_walkers measurements, and at g4acc_flush,
+ * g4acc_sync, g4acc_finalize, g4acc_get_g4 and g4acc_sum_to; g4acc_reset discards it. Results
+ * equal g4acc_accumulate_batch over the same measurements in the same chunks. */
+int g4acc_accumulate_queued(g4acc_handle* h, const g4acc_walker* walker, int memory);
+/* Runs the queued measurements now. */
+int g4acc_flush(g4acc_handle* h);
+/* synchronizeCopy() for queued measurements: returns once the caller's buffers have been copied
+ * (device copies complete); does not run the queue. */
+int g4acc_sync_copy(g4acc_handle* h);
+/* finalize() (tp_accumulator_gpu.hpp:471-484): runs the queue and waits for all work. */
 int g4acc_finalize(g4acc_handle* h);
-/* synchronizeCopy()/syncStreams(): after this returns, caller buffers may be reused. */
+/* syncStreams(): runs the queue and waits for all work; caller buffers may then be reused. */
 int g4acc_sync(g4acc_handle* h);
 /* get_G4()[channel_index] (tp_accumulator_cpu.hpp:871-877): copies n_complex = g4acc_g4_size()
  * complex doubles in the reference order. */

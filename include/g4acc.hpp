@@ -84,8 +84,10 @@ public:
   using TpComplex = std::complex<double>;
   using TpGreensFunction = std::vector<TpComplex>;  // reference order, first index fastest
 
-  TpAccumulator(const Geometry& geometry, const g4acc_options& options)
-      : geometry_(geometry) {
+  // queued = true: accumulate() hands each measurement to the handle's queue, which runs as one
+  // batch of options.max_walkers measurements (g4acc_accumulate_queued).
+  TpAccumulator(const Geometry& geometry, const g4acc_options& options, bool queued = false)
+      : geometry_(geometry), queued_(queued) {
     const g4acc_geometry g = geometry_.view();
     throw_on_error(g4acc_create(&g, &options, &handle_), g4acc_last_error(nullptr));
   }
@@ -96,7 +98,9 @@ public:
   TpAccumulator(const TpAccumulator&) = delete;
   TpAccumulator& operator=(const TpAccumulator&) = delete;
   TpAccumulator(TpAccumulator&& other) noexcept
-      : geometry_(std::move(other.geometry_)), handle_(std::exchange(other.handle_, nullptr)) {}
+      : geometry_(std::move(other.geometry_)),
+        queued_(other.queued_),
+        handle_(std::exchange(other.handle_, nullptr)) {}
 
   // Zeroes G4 (tp_accumulator_cpu.hpp:224-229). The loop index is accepted for signature parity.
   void resetAccumulation(unsigned int /*dca_loop*/ = 0) { check(g4acc_reset(handle_)); }
@@ -128,18 +132,18 @@ public:
     }
     w.sign = static_cast<double>(sign);
     const int kind = memory == G4ACC_HOST ? G4ACC_HOST : G4ACC_DEVICE_M;
-    check(g4acc_accumulate(handle_, &w, kind));
+    check(queued_ ? g4acc_accumulate_queued(handle_, &w, kind) : g4acc_accumulate(handle_, &w, kind));
     return flops(w.n[0], w.n[1]);
   }
 
-  // Waits for queued work (tp_accumulator_gpu.hpp:471-484).
+  // Runs the queue and waits for all work (tp_accumulator_gpu.hpp:471-484).
   void finalize() { check(g4acc_finalize(handle_)); }
   // After this returns the M matrices passed to accumulate may be overwritten
-  // (ctaux_accumulator.hpp:394).
-  void synchronizeCopy() { check(g4acc_sync(handle_)); }
+  // (ctaux_accumulator.hpp:394). Queued measurements keep waiting for their batch.
+  void synchronizeCopy() { check(g4acc_sync_copy(handle_)); }
   template <class T>
   void syncStreams(const T&) {
-    synchronizeCopy();
+    check(g4acc_sync(handle_));
   }
 
   // get_G4() (tp_accumulator_cpu.hpp:871-877): one vector per channel, in the reference order.
@@ -177,6 +181,7 @@ private:
   }
 
   Geometry geometry_;
+  bool queued_ = false;
   g4acc_handle* handle_ = nullptr;
   std::array<std::vector<double>, 2> tau_;
   std::array<std::vector<int>, 2> site_, band_;

@@ -149,6 +149,30 @@ int main(int argc, char** argv) {
     if (!(linf < tol))
       return fail("G4 mismatch");
   }
+  // The same measurements through the queued drop-in: one accumulate() per measurement, run in
+  // batches of three inside the handle.
+  {
+    g4acc_options qopt = opt;
+    qopt.max_walkers = 3;
+    g4acc::TpAccumulator queued(geo, qopt, true);
+    queued.resetAccumulation(0);
+    for (int i = 0; i < n_meas; ++i) {
+      queued.accumulate(ms[i].M, ms[i].config, ms[i].sign);
+      queued.synchronizeCopy();
+    }
+    queued.finalize();
+    const auto& Q = queued.get_G4();
+    for (int c = 0; c < n_ch; ++c) {
+      double num = 0, den = 0;
+      for (int i = 0; i < n_g4; ++i) {
+        num = std::max(num, std::abs(Q[c][i] - expected[c][i]));
+        den = std::max(den, std::abs(expected[c][i]));
+      }
+      std::printf("queued channel %d l_inf %.3e\n", geo.channels[c], num / den);
+      if (!(num / den < tol))
+        return fail("queued G4 mismatch");
+    }
+  }
   // get_G4 on the emptied source: std::logic_error (tp_accumulator_cpu.hpp:874-875).
   try {
     other.get_G4();

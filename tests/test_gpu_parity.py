@@ -104,6 +104,42 @@ def test_accumulate_matches_oracle(oracle_lib, name, precision, sum_mode, tol):
         assert err < tol
 
 
+@pytest.mark.parametrize("name", ["1band_4x4_4q", "3band_2x2"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("memory", ["host", "device"])
+def test_queued_accumulate_equals_batch(oracle_lib, name, precision, memory):
+    """One accumulate per measurement through the handle's queue (max_walkers = 4) gives the
+    same G4 as accumulate_batch over the same chunks; the caller's buffers are free after
+    sync_copy (host arrays are overwritten right after the call)."""
+    import torch
+    from dca_amd.devpool import DevicePool
+    wl = case(name)
+    g = wl.geometry
+    ms = wl.draw(11)
+    bat = g4acc.TpAccumulator(g, precision=precision, max_walkers=4, max_vertices=wl.n_max)
+    bat.resetAccumulation()
+    bat.accumulate_batch(ms)
+    que = g4acc.TpAccumulator(g, precision=precision, max_walkers=4, max_vertices=wl.n_max)
+    que.resetAccumulation()
+    if memory == "host":
+        for meas in ms:
+            copy = synth.Measurement(n=list(meas.n), M=[m.copy() for m in meas.M],
+                                     tau=[t.copy() for t in meas.tau], site=[x.copy() for x in meas.site],
+                                     band=[x.copy() for x in meas.band], sign=meas.sign)
+            que.accumulate_queued(copy)
+            for m in copy.M:
+                m[...] = np.nan  # the queue holds its own copy
+    else:
+        pool = DevicePool(torch, ms)
+        for w in pool.walkers:
+            que.accumulate_queued(None, memory=g4acc.DEVICE, walker=w)
+            que.sync_copy()
+    ref = oracle_g4(oracle_lib, g, ms)
+    for r, a, b in zip(ref, bat.get_G4(), que.get_G4()):
+        assert np.array_equal(a, b)
+        assert linf(r, b) < (1e-12 if precision == "fp64" else 1e-6)
+
+
 @pytest.mark.parametrize("name", ["1band_3x3_shift2", "3band_2x2"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_batched_equals_sequential(oracle_lib, name, precision):
