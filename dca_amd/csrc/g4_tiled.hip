@@ -25,6 +25,12 @@
 
 namespace g4acc {
 
+// Two values read together from LDS (one 16-byte read for complex64).
+template <typename C>
+struct alignas(2 * sizeof(C)) CPair {
+  C a, b;
+};
+
 // UV[((m * 4 + plane) * nqv + qv) * P + p], planes: a(u0-u1), a(u0+u1), v0-v1, v0+v1.
 template <typename Real>
 __global__ void __launch_bounds__(256) uv_kernel(TiledArgs<Real> a) {
@@ -74,9 +80,11 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
   constexpr int kY = (1 << PH_MAGNETIC) | (1 << PH_CHARGE) | (1 << 1);
   const bool needY = kStatic ? (MASK & kY) != 0 : a.need_y;
   const bool needZ = kStatic ? (MASK & (1 << PP_UP_DOWN)) != 0 : a.need_z;
-  C* Yl = lds;                        // [spin][b][a], a fastest
-  C* Zl = Yl + (needY ? 2 * bs : 0);
-  C* Ul = Zl + (needZ ? 2 * bs : 0);  // [plane][qi][k]: a(u0-u1), a(u0+u1), v0-v1, v0+v1
+  // Both spins of a staged element side by side, and the two factors a thread reads together.
+  using P2 = CPair<C>;
+  P2* Yl = reinterpret_cast<P2*>(lds);  // [b][a] (a fastest): {spin 0, spin 1}
+  P2* Zl = Yl + (needY ? bs : 0);
+  P2* Ul = Zl + (needZ ? bs : 0);       // [side][qi][k]: side 0 {a(u0-u1), a(u0+u1)}, 1 {v0-v1, v0+v1}
   const size_t xoff = nc * static_cast<size_t>(w1 + e) + D * (nc * static_cast<size_t>(w2 + e));
   const size_t yoff = nc * static_cast<size_t>(w2 + nu + e) + D * (nc * static_cast<size_t>(w1 + nu + e));
   const size_t zoff = nc * static_cast<size_t>(nu + nw - 1 - w1 + e) +
@@ -110,6 +118,7 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
   const int n_uv = 4 * QG * nc;
   const int u_plane = threadIdx.x / (QG * nc), u_qi = (threadIdx.x / nc) % QG, u_k = threadIdx.x % nc;
   const bool u_ok = threadIdx.x < n_uv;
+  const int u_dst = 2 * (((u_plane >> 1) * QG + u_qi) * nc + u_k) + (u_plane & 1);
   const size_t plane = static_cast<size_t>(nqv) * P;
   const size_t u_src = u_plane * plane + static_cast<size_t>(qv[0] + u_qi) * P + u_k +
                        static_cast<size_t>(nc) * (u_plane < 2 ? w1 : w2);
@@ -193,18 +202,14 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
       const int idx = threadIdx.x + TT * j;
       if (idx < nc * nc) {
         const int li = (idx / nc) * ld + (idx % nc);
-        if (needY) {
-          Yl[li] = pre[0][j];
-          Yl[bs + li] = pre[1][j];
-        }
-        if (needZ) {
-          Zl[li] = pre[2][j];
-          Zl[bs + li] = pre[3][j];
-        }
+        if (needY)
+          Yl[li] = P2{pre[0][j], pre[1][j]};
+        if (needZ)
+          Zl[li] = P2{pre[2][j], pre[3][j]};
       }
     }
     if (u_valid)
-      Ul[threadIdx.x] = pre_uv;
+      reinterpret_cast<C*>(Ul)[u_dst] = pre_uv;
     C xw[2][kPrefetchX ? PPT : 1];
     if constexpr (kPrefetchX) {
 #pragma unroll
@@ -240,11 +245,12 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
       for (int qi = 0; qi < QG; ++qi) {
         if (!qok[qi])
           continue;
-        const C* uq = Ul + qi * nc;
+        const P2* uq = Ul + qi * nc;
         if constexpr (kChunk) {
           const int jj = kChunk ? j : 0, qq = kChunk ? qi : 0;
           if (needY) {
-            const C y0 = Yl[yi[j][qi]], y1 = Yl[bs + yi[j][qi]];
+            const P2 yy = Yl[yi[j][qi]];
+            const C y0 = yy.a, y1 = yy.b;
             cfma(xs[jj][qq], x0, y0);
             cfma(xs[jj][qq], x1, y1);
             if (has_trans) {
@@ -253,19 +259,23 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
             }
           }
           if (needZ) {
-            const C z0 = Zl[zi[j][qi]], z1 = Zl[bs + zi[j][qi]];
-            cfma(pz[jj][qq], x0, z1);
-            cfma(pz[jj][qq], x1, z0);
+            const P2 zz = Zl[zi[j][qi]];
+            cfma(pz[jj][qq], x0, zz.b);
+            cfma(pz[jj][qq], x1, zz.a);
           }
-          if (has_mag)
-            cfma(dm[jj][qq], uq[k1[j]], uq[2 * QG * nc + k2[j]]);
-          if (has_chg)
-            cfma(dc[jj][qq], uq[QG * nc + k1[j]], uq[3 * QG * nc + k2[j]]);
+          if (has_mag || has_chg) {
+            const P2 uu = uq[k1[j]], vv = uq[QG * nc + k2[j]];
+            if (has_mag)
+              cfma(dm[jj][qq], uu.a, vv.a);
+            if (has_chg)
+              cfma(dc[jj][qq], uu.b, vv.b);
+          }
         }
         else {
           C cross{0, 0}, ppz{0, 0}, trans{0, 0};
           if (needY) {
-            const C y0 = Yl[yi[j][qi]], y1 = Yl[bs + yi[j][qi]];
+            const P2 yy = Yl[yi[j][qi]];
+            const C y0 = yy.a, y1 = yy.b;
             cross = cmul(x0, y0);
             cfma(cross, x1, y1);
             if (has_trans) {
@@ -274,9 +284,9 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
             }
           }
           if (needZ) {
-            const C z0 = Zl[zi[j][qi]], z1 = Zl[bs + zi[j][qi]];
-            ppz = cmul(x0, z1);
-            cfma(ppz, x1, z0);
+            const P2 zz = Zl[zi[j][qi]];
+            ppz = cmul(x0, zz.b);
+            cfma(ppz, x1, zz.a);
           }
 #pragma unroll
           for (int c = 0; c < NCH; ++c) {
@@ -284,11 +294,11 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
             C t;
             if (ch == PH_MAGNETIC) {
               t = csub(C{0, 0}, cross);
-              cfma(t, uq[k1[j]], uq[2 * QG * nc + k2[j]]);
+              cfma(t, uq[k1[j]].a, uq[QG * nc + k2[j]].a);
             }
             else if (ch == PH_CHARGE) {
               t = csub(C{0, 0}, cross);
-              cfma(t, uq[QG * nc + k1[j]], uq[3 * QG * nc + k2[j]]);
+              cfma(t, uq[k1[j]].b, uq[QG * nc + k2[j]].b);
             }
             else if (ch == PP_UP_DOWN)
               t = ppz;
