@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 from typing import Sequence
 
 import numpy as np
@@ -67,6 +68,25 @@ class Walker_c(ctypes.Structure):
 _lib = None
 
 
+def _share_hip_runtime():
+    """Keeps one HIP runtime per process when torch is used beside the library.
+
+    The torch wheel bundles its own libamdhip64.so / libhsa-runtime64.so (sonames .so.7 / .so.1)
+    and its libraries ask for them by the unversioned names. Loaded after torch, libg4acc.so binds
+    to torch's copy through the shared soname. Loaded first, it would bring in the system runtime
+    under the versioned name only, and a later `import torch` would load a second runtime into
+    the process (whose device enumeration then fails). Loading the runtime under the unversioned
+    names first makes a later torch bind to the same one.
+    """
+    if "torch" in sys.modules:
+        return
+    for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+        try:
+            ctypes.CDLL(name, mode=ctypes.RTLD_GLOBAL)
+        except OSError:
+            pass  # libg4acc.so then resolves the runtime by its own dependencies
+
+
 def load_library(path: str = LIB_PATH):
     """Loads libg4acc.so; raises if it has not been built."""
     global _lib
@@ -75,6 +95,7 @@ def load_library(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise ImportError(f"{path} is missing: build it with dca_amd/build_lib.sh "
                           "(or __graft_entry__.build()); there is no CPU fallback")
+    _share_hip_runtime()
     L = ctypes.CDLL(path)
     vp, sz, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
     sig = {
@@ -310,7 +331,7 @@ class TpAccumulator:
         return self.lib.g4acc_bytes(self._h)
 
     def set_tuning(self, **params):
-        """Launch-shape tuning: mtog_tw, g4_qg, g4_tile_order, g4_mfma, graph (include/g4acc.h)."""
+        """Launch-shape tuning: mtog_tw, mtog_mfma, g4_qg, g4_tile_order, g4_mfma, graph (include/g4acc.h)."""
         for k, v in params.items():
             self._check(self.lib.g4acc_set_tuning(self._h, k.encode(), int(v)))
 
