@@ -3,6 +3,7 @@
 For a workload and a batch size it measures, with the library's own HIP-event stage timing on
 its stream, every candidate of each tunable of include/g4acc.h (g4acc_set_tuning):
   mtog_tw        frequencies per thread of the fused single-band M -> G kernel (M -> G stage)
+  mtog_mfma      right half of that kernel on the matrix cores (complex64, Nc <= 16)
   g4_qg          momenta per G4 workgroup                                        (G4 stage)
   g4_tile_order  natural, XCD-aware super-tile or diagonal-strip tile schedule   (G4 stage)
   g4_mfma        rank-one G4 terms on the matrix cores or the vector ALUs         (G4 stage)
@@ -26,6 +27,7 @@ TABLE = os.path.join(HERE, "tuning.json")
 
 CANDIDATES = {
     "mtog_tw": [0, 1, 2, 4, 8],
+    "mtog_mfma": [0, 1],
     "g4_qg": [0, 1, 2, 4],
     "g4_tile_order": [0, 1, 2],
     "g4_mfma": [0, 1],
@@ -76,15 +78,18 @@ def tune(cfg: int, walkers: int, precision: str, reps: int = 4, log=print) -> di
     acc.resetAccumulation()
     _stage_times(acc, batches, 1)  # warm up
     measured = {"mtog": {}, "g4": {}}
-    best_tw, best_t = 0, float("inf")
-    for tw in CANDIDATES["mtog_tw"]:
-        acc.set_tuning(mtog_tw=tw)
+    best_tw, best_mm, best_t = 0, 0, float("inf")
+    # The matrix-core kernel has one shape (mtog_tw does not apply to it); where it does not
+    # apply (complex128, Nc > 16) the setting falls back to the vector kernel and measures alike.
+    for mm, tw in [(0, tw) for tw in CANDIDATES["mtog_tw"]] + [(1, 0)]:
+        acc.set_tuning(mtog_tw=tw, mtog_mfma=mm)
         t, _ = _stage_times(acc, batches, reps)
-        measured["mtog"][f"mtog_tw={tw}"] = round(t, 4)
-        log(f"  M->G  mtog_tw={tw}: {t:.3f} ms")
+        name = "mtog_mfma=1" if mm else f"mtog_tw={tw}"
+        measured["mtog"][name] = round(t, 4)
+        log(f"  M->G  {name}: {t:.3f} ms")
         if t < best_t:
-            best_tw, best_t = tw, t
-    acc.set_tuning(mtog_tw=best_tw)
+            best_tw, best_mm, best_t = tw, mm, t
+    acc.set_tuning(mtog_tw=best_tw, mtog_mfma=best_mm)
     best_g4, best_g4_t = (0, 0, 1), float("inf")
     for qg, order, mf in itertools.product(CANDIDATES["g4_qg"], CANDIDATES["g4_tile_order"],
                                            CANDIDATES["g4_mfma"]):
@@ -95,7 +100,7 @@ def tune(cfg: int, walkers: int, precision: str, reps: int = 4, log=print) -> di
         if t < best_g4_t:
             best_g4, best_g4_t = (qg, order, mf), t
     acc.close()
-    return {"params": {"mtog_tw": best_tw, "g4_qg": best_g4[0], "g4_tile_order": best_g4[1],
+    return {"params": {"mtog_tw": best_tw, "mtog_mfma": best_mm, "g4_qg": best_g4[0], "g4_tile_order": best_g4[1],
                        "g4_mfma": best_g4[2]},
             "stage_ms": {"mtog": round(best_t, 4), "g4": round(best_g4_t, 4)},
             "measured": measured, "workload": cfg, "when": time.strftime("%Y-%m-%d %H:%M:%S")}

@@ -11,8 +11,10 @@ FLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Wall
        -Xclang -target-feature -Xclang -packed-fp32-ops)
 filter() { grep -v "packed-fp32-ops' is not a recognized feature" >&2 || true; }
 pids=()
+# The matrix-core M -> G kernel keeps its accumulators in VGPRs (no AGPR copies per step).
+extra() { [ "$1" = mtog_fused ] && echo "-mllvm -amdgpu-mfma-vgpr-form=1" || true; }
 for s in "${SRC[@]}"; do
-  /opt/rocm/bin/hipcc "${FLAGS[@]}" -c "$HERE/csrc/$s.hip" -o "$OBJ/$s.o" 2> >(filter) &
+  /opt/rocm/bin/hipcc "${FLAGS[@]}" $(extra "$s") -c "$HERE/csrc/$s.hip" -o "$OBJ/$s.o" 2> >(filter) &
   pids+=($!)
 done
 for p in "${pids[@]}"; do wait "$p"; done

@@ -84,6 +84,7 @@ struct g4acc_handle {
   // Captured launch sequences, one per batch size (G4ACC_GRAPH=0 disables them).
   bool use_graph = true;
   int mtog_tw = 0;                 // tuning: frequencies per thread of the fused M -> G kernel
+  int mtog_mfma = 0;               // tuning: fused right M -> G kernel on the matrix cores
   int g4_qg = 0;                   // tuning: momenta per G4 workgroup (0: register limit)
   int g4_mfma = 0;                 // tuning: rank-one G4 terms on the matrix cores (FP32 G, Nc <= 16)
   std::vector<std::pair<int, hipGraphExec_t>> graphs;  // key: (walkers << 1) | stage
@@ -326,6 +327,7 @@ MtoGArgs<Real> mtog_args(g4acc_handle* h, int n_slots) {
   a.chunk = h->chunk;
   a.fused = h->fused_mtog;
   a.tw = h->mtog_tw;
+  a.mfma = h->mtog_mfma;
   a.perm = h->d_perm;
   a.off = h->d_off;
   a.T = static_cast<Cplx<Real>*>(h->d_T);
@@ -1032,6 +1034,8 @@ int g4acc_set_tuning(g4acc_handle* h, const char* key, int value) {
   const std::string k(key);
   if (k == "mtog_tw" && (value == 0 || value == 1 || value == 2 || value == 4 || value == 8))
     h->mtog_tw = value;
+  else if (k == "mtog_mfma" && (value == 0 || value == 1))
+    h->mtog_mfma = value;
   else if (k == "g4_qg" && (value == 0 || value == 1 || value == 2 || value == 4))
     h->g4_qg = value;
   else if (k == "g4_tile_order" && value >= -1 && value <= 2)

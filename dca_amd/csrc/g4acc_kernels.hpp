@@ -14,6 +14,7 @@ struct MtoGArgs {
   int chunk;              // slots whose S1..S4 workspaces exist at once
   bool fused;             // use the register-fused single-band kernels
   int tw;                 // frequencies per thread in the fused right kernel (0: default)
+  int mfma;               // fused right kernel on the matrix cores (complex64, Nc <= 16)
   const int* perm;        // [n_slots][n_max]
   const int* off;         // [n_slots][n_orb + 1]
   Cplx<Real>* T;          // [n_slots][n_max][nwE]

@@ -12,6 +12,8 @@
 // T[w, j] = exp(i w_w tau_{perm j}) and the site runs of the sorted vertices come from
 // sort_kernel / phase_kernel. Per slot this is 8 nwE N^2 + 8 Nc D N + 8 D N nwE + 8 Nc D^2 flops
 // and one write of G: the E matrix (D x D) of the unfused path is never stored.
+#include <type_traits>
+
 #include "g4acc_common.hpp"
 #include "g4acc_kernels.hpp"
 
@@ -304,6 +306,281 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
   }
 }
 
+typedef float mtog_f32x4 __attribute__((ext_vector_type(4)));
+
+// Steps of four sorted vertices a slot may need (sum over sites of ceil(n_r / 4)) that the
+// matrix-core right kernel tracks; vertex capacities beyond mtog_mfma_max_vertices(nc) use the
+// vector kernel.
+constexpr int mtog_mfma_max_steps() { return 128; }
+constexpr int mtog_mfma_max_vertices(int nc) { return 4 * (mtog_mfma_max_steps() - nc); }
+
+// Right half on the matrix cores (complex64, Nc <= 16). Two waves share a tile of 16
+// frequencies w2 by 16 rows p1; with SPLIT = 2 each takes half of the sites. For its sites r a
+// wave accumulates
+//     E_r[w2][p1] = sum_{j in site r} conj(T[w2, j]) A[p1, j]
+// with v_mfma_f32_16x16x4f32: one step takes four sorted vertices of one site (the last step of
+// a site is padded with zeros) and issues four real products. Lane l supplies the operands
+// conj-T[w0 + l%16][j0 + l/16] and A[p0 + l%16][j0 + l/16], and ends up holding
+// E_r[w0 + 4(l/16) + i][p0 + l%16] (i = 0..3). The two waves of a pair then swap halves through
+// LDS so that each holds all sites for two of the four i: the site DFT and the FP64 epilogue are
+// lane-local, and each store of one (i, k2) writes 16 consecutive rows (128 bytes) in each of four
+// columns of G. Splitting the sites halves the registers that hold them (four waves per SIMD for
+// Nc = 16). Operands are loaded PF steps ahead of their use; the workgroup walks all frequency
+// tiles of its rows, so the step table is built once per slot.
+template <int NC, bool SEP>
+__global__ void __launch_bounds__(256) mtog_right_mfma(const SlotDesc* __restrict__ desc, Geo g,
+                                                       int n_max, int slot0,
+                                                       const int* __restrict__ off,
+                                                       const float2* __restrict__ T,
+                                                       const float2* __restrict__ A,
+                                                       const float2* __restrict__ dft,
+                                                       const double2* __restrict__ bphase,
+                                                       const double2* __restrict__ G0,
+                                                       const int* __restrict__ kmap,
+                                                       float2* __restrict__ G) {
+  using C = float2;
+  constexpr int L = SEP ? grid_side<NC>() : 0;
+  constexpr int PF = 4;
+  constexpr int kSteps = mtog_mfma_max_steps();
+  constexpr int SPLIT = NC == 16 ? 2 : 1;  // waves per tile
+  constexpr int NH = NC / SPLIT;           // sites per wave
+  constexpr int NI = 4 / SPLIT;            // frequencies per lane in the epilogue
+  constexpr int TPW = 4 / SPLIT;           // row tiles per workgroup
+  __shared__ C Dl[SEP ? 1 : NC * NC];
+  __shared__ double2 G0l[NC * 16];
+  __shared__ int Kl[NC];
+  __shared__ int So[NC + 1];  // vertex range of each site
+  __shared__ int Sp[NC + 1];  // first step of each site (steps of four vertices, per site)
+  // Site halves in transit: [wave][NH sites][NI i][re, im][lane].
+  __shared__ float Xl[SPLIT > 1 ? 4 * 64 * NH * NI * 2 : 1];
+  const int slot = slot0 + blockIdx.z;
+  const int s = slot & 1;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int half = SPLIT > 1 ? (wave & 1) : 0;
+  const int p0 = blockIdx.x * (16 * TPW) + 16 * (SPLIT > 1 ? wave >> 1 : wave);
+  const int* O = off + static_cast<size_t>(slot) * (NC + 1);
+  const C* Ts = T + static_cast<size_t>(slot) * g.nwE * n_max;
+  const C* Ab = A + static_cast<size_t>(blockIdx.z) * g.D * n_max;
+
+  if constexpr (!SEP)
+    for (int i = threadIdx.x; i < NC * NC; i += 256)
+      Dl[i] = dft[i];
+  if (threadIdx.x < NC)
+    Kl[threadIdx.x] = kmap ? kmap[threadIdx.x] : threadIdx.x;
+  if (threadIdx.x <= NC)
+    So[threadIdx.x] = O[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int r = 0; r < NC; ++r) {
+      Sp[r] = acc;
+      acc += (So[r + 1] - So[r] + 3) >> 2;
+    }
+    Sp[NC] = acc;
+  }
+  __syncthreads();
+  // This wave's sites [r_lo, r_lo + NH) and their steps [q_lo, q_lo + n_steps).
+  const int r_lo = half * NH;
+  const int q_lo = Sp[r_lo], n_steps = Sp[r_lo + NH] - q_lo;
+  // Step table across the lanes: step q_lo + q (q = lane + 64 h) is entry h of lane q % 64,
+  // packed as first vertex (bits 0-19), vertices in the step (20-22) and, for the last step of a
+  // site, 1 + (site - r_lo) (24-28): the accumulated tile is moved to that site after the step.
+  int tab[kSteps / 64];
+#pragma unroll
+  for (int h = 0; h < kSteps / 64; ++h) {
+    const int q = lane + 64 * h;
+    int e = 0;
+    if (q < n_steps) {
+      const int qg = q_lo + q;
+      int r = r_lo;
+      while (Sp[r + 1] <= qg)
+        ++r;
+      const int base = So[r] + 4 * (qg - Sp[r]);
+      const int cnt = min(4, So[r + 1] - base);
+      e = base | (cnt << 20) | (qg == Sp[r + 1] - 1 ? (r - r_lo + 1) << 24 : 0);
+    }
+    tab[h] = e;
+  }
+  auto entry = [&](int q) {  // q wave-uniform
+    int e = 0;
+#pragma unroll
+    for (int h = 0; h < kSteps / 64; ++h) {
+      const int v = __builtin_amdgcn_readlane(tab[h], q & 63);
+      e = (q >> 6) == h ? v : e;
+    }
+    return q < n_steps ? e : 0;
+  };
+  const int jl = lane >> 4, pl = p0 + (lane & 15);
+  const bool p_ok = pl < g.D;
+  const int pc = p_ok ? pl : g.D - 1;
+
+  const int n_wt = (g.nwE + 15) / 16;
+  for (int wt = blockIdx.y; wt < n_wt; wt += gridDim.y) {
+    const int w0 = wt * 16;
+    __syncthreads();  // the previous tile's epilogue has read G0l and Xl
+    for (int i = threadIdx.x; i < NC * 16; i += 256) {
+      const int k = i % NC, w = w0 + i / NC;
+      G0l[i] = w < g.nwE ? G0[s + 2 * (k + static_cast<size_t>(NC) * w)] : double2{0, 0};
+    }
+    const int wl = w0 + (lane & 15);
+    const bool w_ok = wl < g.nwE;
+    // Branch-free loads at clamped (in-bounds) addresses; lanes past the step's vertices are
+    // zeroed when the operands are consumed.
+    const int wc = w_ok ? wl : g.nwE - 1;
+    auto load = [&](int e, C& ta, C& aa) {
+      const int v = (e & 0xfffff) + jl;
+      const int vc = jl < ((e >> 20) & 7) ? v : 0;
+      ta = Ts[wc + static_cast<size_t>(g.nwE) * vc];
+      aa = Ab[pc + static_cast<size_t>(g.D) * vc];
+    };
+    // The products of the current site accumulate in one register tile (cr, ci); after the last
+    // step of a site the tile is moved to that site's slot (er, ei). The ring of loaded operands
+    // is indexed statically (the loop is unrolled by PF), so a load's destination is never moved
+    // while it is in flight.
+    mtog_f32x4 er[NH], ei[NH];
+#pragma unroll
+    for (int r = 0; r < NH; ++r)
+      er[r] = ei[r] = mtog_f32x4{0, 0, 0, 0};
+    mtog_f32x4 cr{0, 0, 0, 0}, ci{0, 0, 0, 0};
+    C qt[PF], qa[PF];
+    int qe[PF];
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      qe[k] = entry(k);
+      load(qe[k], qt[k], qa[k]);
+    }
+    for (int g0 = 0; g0 < n_steps; g0 += PF) {
+#pragma unroll
+      for (int k = 0; k < PF; ++k) {
+        if (g0 + k >= n_steps)
+          break;
+        const int ce = qe[k];
+        const bool lok = jl < ((ce >> 20) & 7);
+        const bool tok = lok && w_ok, aok = lok && p_ok;
+        const C ct{tok ? qt[k].x : 0.f, tok ? qt[k].y : 0.f};
+        const C ca{aok ? qa[k].x : 0.f, aok ? qa[k].y : 0.f};
+        qe[k] = entry(g0 + k + PF);
+        load(qe[k], qt[k], qa[k]);
+        // E += conj(T) A = (Tx Ax + Ty Ay) + i (Tx Ay - Ty Ax)
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(ct.x, ca.x, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(ct.x, ca.y, ci, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(ct.y, ca.y, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(-ct.y, ca.x, ci, 0, 0, 0);
+        const int fs = ce >> 24;
+        if (fs != 0) {
+          switch (fs - 1) {  // a branch, not per-site selects on every step
+#define MTOG_SITE_STORE(R)  \
+  case R:                   \
+    if constexpr (R < NH) { \
+      er[R] = cr;           \
+      ei[R] = ci;           \
+    }                       \
+    break;
+            MTOG_SITE_STORE(0) MTOG_SITE_STORE(1) MTOG_SITE_STORE(2) MTOG_SITE_STORE(3)
+            MTOG_SITE_STORE(4) MTOG_SITE_STORE(5) MTOG_SITE_STORE(6) MTOG_SITE_STORE(7)
+            MTOG_SITE_STORE(8) MTOG_SITE_STORE(9) MTOG_SITE_STORE(10) MTOG_SITE_STORE(11)
+            MTOG_SITE_STORE(12) MTOG_SITE_STORE(13) MTOG_SITE_STORE(14) MTOG_SITE_STORE(15)
+#undef MTOG_SITE_STORE
+            default: break;
+          }
+          cr = ci = mtog_f32x4{0, 0, 0, 0};
+        }
+      }
+    }
+    // Gather all sites for this lane's NI frequencies: i = NI * half + ii.
+    float xr[NC][NI], xi[NC][NI];
+    if constexpr (SPLIT > 1) {
+      // Send this wave's sites at the partner's frequencies (i = NI (1 - half) + ii); receive the
+      // partner's sites at this wave's frequencies (i = NI half + ii). Indices stay static: the
+      // half only selects values.
+      float* mine = Xl + wave * (NH * NI * 2 * 64) + lane;  // lane fastest: no bank conflicts
+      const float* theirs = Xl + (wave ^ 1) * (NH * NI * 2 * 64) + lane;
+#pragma unroll
+      for (int r = 0; r < NH; ++r)
+#pragma unroll
+        for (int ii = 0; ii < NI; ++ii) {
+          mine[64 * ((r * NI + ii) * 2)] = half ? er[r][ii] : er[r][NI + ii];
+          mine[64 * ((r * NI + ii) * 2 + 1)] = half ? ei[r][ii] : ei[r][NI + ii];
+        }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < NH; ++r)
+#pragma unroll
+        for (int ii = 0; ii < NI; ++ii) {
+          const float own_r = half ? er[r][NI + ii] : er[r][ii];
+          const float own_i = half ? ei[r][NI + ii] : ei[r][ii];
+          const float oth_r = theirs[64 * ((r * NI + ii) * 2)], oth_i = theirs[64 * ((r * NI + ii) * 2 + 1)];
+          xr[r][ii] = half ? oth_r : own_r;  // sites 0..NH-1 belong to half 0
+          xi[r][ii] = half ? oth_i : own_i;
+          xr[NH + r][ii] = half ? own_r : oth_r;
+          xi[NH + r][ii] = half ? own_i : oth_i;
+        }
+    }
+    else {
+#pragma unroll
+      for (int r = 0; r < NC; ++r)
+#pragma unroll
+        for (int ii = 0; ii < NI; ++ii) {
+          xr[r][ii] = er[r][ii];
+          xi[r][ii] = ei[r][ii];
+        }
+    }
+    if (!p_ok)
+      continue;  // every wave walks the same tiles (barriers above)
+    // Epilogue per held frequency: site DFT, band phases, G = -G0 M~ G0 + beta G0 delta (FP64).
+    const int p1 = pl;
+    const int k1 = p1 % NC, w1 = p1 / NC;
+    const double2 ga = G0[s + 2 * (k1 + static_cast<size_t>(NC) * w1)];
+    const double inv_nc = 1.0 / NC;
+    C* Gs = G + static_cast<size_t>(slot) * g.D * g.D + p1;
+#pragma unroll
+    for (int ii = 0; ii < NI; ++ii) {
+      const int w2 = w0 + 4 * jl + NI * half + ii;
+      if (w2 >= g.nwE)
+        break;
+      C e[NC];
+#pragma unroll
+      for (int r = 0; r < NC; ++r)
+        e[r] = C{xr[r][ii], xi[r][ii]};
+      if constexpr (L > 0)
+        site_dft_2d<L, float>(e);
+      else {
+        C f[NC];
+#pragma unroll
+        for (int k2 = 0; k2 < NC; ++k2) {
+          C acc{0, 0};
+#pragma unroll
+          for (int r = 0; r < NC; ++r)
+            cfmac(acc, e[r], Dl[k2 + NC * r]);
+          f[k2] = acc;
+        }
+#pragma unroll
+        for (int k2 = 0; k2 < NC; ++k2)
+          e[k2] = f[k2];
+      }
+#pragma unroll
+      for (int kk = 0; kk < NC; ++kk) {
+        const int k2 = L > 0 ? Kl[kk] : kk;
+        double2 m = to_d(e[kk]);
+        m.x *= inv_nc;
+        m.y *= inv_nc;
+        if (bphase)
+          m = cmulc(cmul(m, bphase[k1]), bphase[k2]);
+        double2 v = cmul(cmul(ga, m), G0l[k2 + NC * (w2 - w0)]);
+        v.x = -v.x;
+        v.y = -v.y;
+        const int p2 = k2 + NC * w2;
+        if (p1 == p2) {
+          v.x = fma(ga.x, g.beta, v.x);
+          v.y = fma(ga.y, g.beta, v.y);
+        }
+        Gs[static_cast<size_t>(g.D) * p2] = from_d<C>(v);
+      }
+    }
+  }
+}
+
 namespace {
 
 template <typename Real, int NC, int TW>
@@ -314,8 +591,24 @@ void launch_fused_nc(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st) {
   const Geo& g = a.g;
   hipLaunchKernelGGL((mtog_left_fused<Real, NC>), dim3((g.nwE * a.n_max + 255) / 256, 1, ns), dim3(256), 0,
                      st, a.desc, g, a.n_max, s0, a.perm, a.off, a.T, a.dft, a.A);
+  // Matrix-core right kernel (complex64): Nc <= 8, or 16 sites of a 4x4 torus (the dense
+  // 16-point DFT epilogue would not fit in registers), with its step table in range.
+  const bool sep = kSep && a.grid_L == grid_side<NC>() && a.kmap;
+  if constexpr (std::is_same<Real, float>::value && NC <= 16) {
+    if (a.mfma && (NC <= 8 || sep) && a.n_max <= mtog_mfma_max_vertices(NC)) {
+      const int rows = NC == 16 ? 32 : 64;  // p1 rows per workgroup (two waves per tile for Nc = 16)
+      const dim3 grid((g.D + rows - 1) / rows, 1, ns);  // each workgroup walks all frequency tiles
+      if (sep)
+        hipLaunchKernelGGL((mtog_right_mfma<NC, kSep>), grid, dim3(256), 0, st, a.desc, g, a.n_max, s0, a.off,
+                           a.T, a.A, a.dft, a.bphase, a.G0, a.kmap, a.G);
+      else if constexpr (NC <= 8)
+        hipLaunchKernelGGL((mtog_right_mfma<NC, false>), grid, dim3(256), 0, st, a.desc, g, a.n_max, s0, a.off,
+                           a.T, a.A, a.dft, a.bphase, a.G0, nullptr, a.G);
+      return;
+    }
+  }
   const dim3 grid((g.D + 63) / 64, (g.nwE + 4 * TW - 1) / (4 * TW), ns);
-  if (kSep && a.grid_L == grid_side<NC>() && a.kmap)
+  if (sep)
     hipLaunchKernelGGL((mtog_right_fused<Real, NC, TW, kSep>), grid, dim3(256), 0, st, a.desc, g, a.n_max, s0,
                        a.off, a.T, a.A, a.dft, a.bphase, a.G0, a.kmap, a.G);
   else

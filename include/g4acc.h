@@ -169,7 +169,9 @@ int g4acc_set_timing(g4acc_handle* h, int enable);
 
 /* Launch-shape tuning (no counterpart in the reference, whose launch shapes are fixed; set by
  * the autotuner's table, dca_amd/autotune.py). Keys: "mtog_tw" (frequencies per thread of the
- * fused single-band M -> G kernel: 0 default, 1, 2, 4, 8), "g4_qg" (momenta per G4 workgroup:
+ * fused single-band M -> G kernel: 0 default, 1, 2, 4, 8), "mtog_mfma" (1: its right half on the
+ * matrix cores where available, FP32 G, Nc <= 8 or a 4x4 torus, max_vertices <= 4 (128 - Nc);
+ * 0: vector ALUs), "g4_qg" (momenta per G4 workgroup:
  * 0 = as many as registers allow, 1, 2, 4), "g4_tile_order" (-1 by batch size, 0 natural,
  * 1 XCD-aware super-tiles, 2 XCD-aware diagonal strips with their mirror images), "g4_mfma"
  * (1: rank-one G4 terms on the matrix cores where available, FP32 G and Nc <= 16; 0: vector ALUs),

@@ -319,6 +319,25 @@ def test_fused_equals_unfused_mtog(name, precision, monkeypatch):
         assert err < (1e-14 if precision == "fp64" else 1e-6)
 
 
+@pytest.mark.parametrize("name,dense", [("1band_2x2", False), ("1band_2x2", True),
+                                        ("1band_4x4_4q", False), ("1band_8site", False)])
+def test_mfma_right_mtog_matches_fp64(name, dense, monkeypatch):
+    """The matrix-core right M -> G kernel (complex64; separable and dense site DFT) gives the G
+    of the FP64 four-stage kernels to complex64 accuracy."""
+    wl = case(name)
+    if dense:
+        monkeypatch.setenv("G4ACC_DENSE_DFT", "1")
+    mf = g4acc.TpAccumulator(wl.geometry, precision="fp32", max_vertices=wl.n_max, tuned=False)
+    mf.set_tuning(mtog_mfma=1)
+    monkeypatch.setenv("G4ACC_UNFUSED_MTOG", "1")
+    ref = g4acc.TpAccumulator(wl.geometry, precision="fp64", max_vertices=wl.n_max)
+    for meas in wl.draw(3):
+        a, b = mf.compute_G(meas), ref.compute_G(meas)
+        err = linf(b, a)
+        print(f"{name} dense={dense} matrix-core M->G vs FP64 G l_inf={err:.3e}", flush=True)
+        assert err < 1e-6
+
+
 @pytest.mark.parametrize("name", ["1band_2x2", "1band_4x4_4q", "1band_6x6", "3band_2x2"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_separable_site_dft_equals_dense(name, precision, monkeypatch):
