@@ -7,6 +7,7 @@ its stream, every candidate of each tunable of include/g4acc.h (g4acc_set_tuning
   g4_qg          momenta per G4 workgroup                                        (G4 stage)
   g4_tile_order  natural, XCD-aware super-tile or diagonal-strip tile schedule   (G4 stage)
   g4_mfma        rank-one G4 terms on the matrix cores or the vector ALUs         (G4 stage)
+  g4_rank1       rank-one terms of the whole batch as one matrix-core GEMM          (G4 stage)
 The stages are independent, so each is tuned on its own (the G4 knobs jointly). Results go to a
 JSON table (dca_amd/tuning.json by default) keyed by the geometry, the precision and the batch
 size; TpAccumulator applies the matching entry when it is constructed.
@@ -31,6 +32,7 @@ CANDIDATES = {
     "g4_qg": [0, 1, 2, 4],
     "g4_tile_order": [0, 1, 2],
     "g4_mfma": [0, 1],
+    "g4_rank1": [0, 1],
 }
 
 
@@ -90,18 +92,23 @@ def tune(cfg: int, walkers: int, precision: str, reps: int = 4, log=print) -> di
         if t < best_t:
             best_tw, best_mm, best_t = tw, mm, t
     acc.set_tuning(mtog_tw=best_tw, mtog_mfma=best_mm)
-    best_g4, best_g4_t = (0, 0, 1), float("inf")
-    for qg, order, mf in itertools.product(CANDIDATES["g4_qg"], CANDIDATES["g4_tile_order"],
-                                           CANDIDATES["g4_mfma"]):
-        acc.set_tuning(g4_qg=qg, g4_tile_order=order, g4_mfma=mf)
+    best_g4, best_g4_t = (0, 0, 1, 0), float("inf")
+    # The two rank-one forms exclude each other: g4_mfma forms them per walker on the matrix
+    # cores inside the tile, g4_rank1 once per batch before it.
+    for qg, order, mf, r1 in itertools.product(CANDIDATES["g4_qg"], CANDIDATES["g4_tile_order"],
+                                               CANDIDATES["g4_mfma"], CANDIDATES["g4_rank1"]):
+        if mf and r1:
+            continue
+        acc.set_tuning(g4_qg=qg, g4_tile_order=order, g4_mfma=mf, g4_rank1=r1)
         _, t = _stage_times(acc, batches, reps)
-        measured["g4"][f"g4_qg={qg},g4_tile_order={order},g4_mfma={mf}"] = round(t, 4)
-        log(f"  G4    g4_qg={qg} g4_tile_order={order} g4_mfma={mf}: {t:.3f} ms")
+        name = f"g4_qg={qg},g4_tile_order={order},g4_mfma={mf},g4_rank1={r1}"
+        measured["g4"][name] = round(t, 4)
+        log(f"  G4    {name.replace(',', ' ')}: {t:.3f} ms")
         if t < best_g4_t:
-            best_g4, best_g4_t = (qg, order, mf), t
+            best_g4, best_g4_t = (qg, order, mf, r1), t
     acc.close()
     return {"params": {"mtog_tw": best_tw, "mtog_mfma": best_mm, "g4_qg": best_g4[0], "g4_tile_order": best_g4[1],
-                       "g4_mfma": best_g4[2]},
+                       "g4_mfma": best_g4[2], "g4_rank1": best_g4[3]},
             "stage_ms": {"mtog": round(best_t, 4), "g4": round(best_g4_t, 4)},
             "measured": measured, "workload": cfg, "when": time.strftime("%Y-%m-%d %H:%M:%S")}
 

@@ -6,13 +6,13 @@ HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 OUT="$HERE/libg4acc.so"
 OBJ="$HERE/_build"
 mkdir -p "$OBJ"
-SRC=(g4acc_api mtog_kernels mtog_fused g4_kernels g4_tiled g4_mfma g4_band)
+SRC=(g4acc_api mtog_kernels mtog_fused g4_kernels g4_tiled g4_mfma g4_band g4_rank1)
 FLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Wall
        -Xclang -target-feature -Xclang -packed-fp32-ops)
 filter() { grep -v "packed-fp32-ops' is not a recognized feature" >&2 || true; }
 pids=()
-# The matrix-core M -> G kernel keeps its accumulators in VGPRs (no AGPR copies per step).
-extra() { [ "$1" = mtog_fused ] && echo "-mllvm -amdgpu-mfma-vgpr-form=1" || true; }
+# The matrix-core M -> G and rank-one kernels keep their accumulators in VGPRs (no AGPR copies).
+extra() { case "$1" in mtog_fused|g4_rank1) echo "-mllvm -amdgpu-mfma-vgpr-form=1" ;; esac; }
 for s in "${SRC[@]}"; do
   /opt/rocm/bin/hipcc "${FLAGS[@]}" $(extra "$s") -c "$HERE/csrc/$s.hip" -o "$OBJ/$s.o" 2> >(filter) &
   pids+=($!)

@@ -62,7 +62,9 @@ __global__ void __launch_bounds__(256) uv_kernel(TiledArgs<Real> a) {
 }
 
 // MASK: compile-time channel set (bit 1 << type), or 0 to read the channel flags at run time.
-template <typename Real, int TT, int PPT, int QG, int NCH, int MASK>
+// RX: the rank-one terms of MAGNETIC and CHARGE come precomputed for the batch (a.R, g4_rank1.hip)
+// and are added at the G4 store; the kernel then stages no factors and forms no rank-one products.
+template <typename Real, int TT, int PPT, int QG, int NCH, int MASK, bool RX>
 __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
   using C = Cplx<Real>;
   extern __shared__ unsigned char smem_raw[];
@@ -122,7 +124,7 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
   const size_t plane = static_cast<size_t>(nqv) * P;
   const size_t u_src = u_plane * plane + static_cast<size_t>(qv[0] + u_qi) * P + u_k +
                        static_cast<size_t>(nc) * (u_plane < 2 ? w1 : w2);
-  const bool u_valid = u_ok && (q0 + u_qi < g.nq);
+  const bool u_valid = !RX && u_ok && (q0 + u_qi < g.nq);
 
   // Channel outputs are FP64 totals. With complex64 G the walker terms are first summed in FP32
   // per term type for FLUSH walkers (relative rounding ~2^-24 per step), so the per-walker work
@@ -263,7 +265,7 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
             cfma(pz[jj][qq], x0, zz.b);
             cfma(pz[jj][qq], x1, zz.a);
           }
-          if (has_mag || has_chg) {
+          if (!RX && (has_mag || has_chg)) {
             const P2 uu = uq[k1[j]], vv = uq[QG * nc + k2[j]];
             if (has_mag)
               cfma(dm[jj][qq], uu.a, vv.a);
@@ -349,8 +351,18 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
                                                (k2[j] + static_cast<size_t>(nc) *
                                                             (w2 + static_cast<size_t>(nw) * qv[qi])));
 #pragma unroll
-      for (int c = 0; c < NCH; ++c)
-        g4_store_t(a.g4[c], idx, tot[j][qi][c], a.sum_mode);
+      for (int c = 0; c < NCH; ++c) {
+        double2 v = tot[j][qi][c];
+        if constexpr (RX) {
+          if (a.r1_slot[c] >= 0) {
+            const size_t plane = static_cast<size_t>(nqv) * P * P;
+            const float2 r = static_cast<const float2*>(a.R)[a.r1_slot[c] * plane + idx];
+            v.x += static_cast<double>(r.x);
+            v.y += static_cast<double>(r.y);
+          }
+        }
+        g4_store_t(a.g4[c], idx, v, a.sum_mode);
+      }
     }
   }
 }
@@ -360,7 +372,16 @@ namespace {
 template <typename Real, int TT, int PPT, int QG, int NCH, int MASK>
 void launch_tiled_fixed(TiledArgs<Real> a, size_t lds, hipStream_t st) {
   const unsigned tiles = tile_schedule(a);
-  hipLaunchKernelGGL((g4_tiled_kernel<Real, TT, PPT, QG, NCH, MASK>), dim3(tiles), dim3(TT), lds, st, a);
+  constexpr bool kRX = std::is_same<Real, float>::value && TT == 256 && MASK != 0 &&
+                       (MASK & ((1 << PH_MAGNETIC) | (1 << PH_CHARGE))) != 0;
+  if constexpr (kRX) {
+    if (a.R) {
+      launch_g4_rank1(a, st);
+      hipLaunchKernelGGL((g4_tiled_kernel<Real, TT, PPT, QG, NCH, MASK, true>), dim3(tiles), dim3(TT), lds, st, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((g4_tiled_kernel<Real, TT, PPT, QG, NCH, MASK, false>), dim3(tiles), dim3(TT), lds, st, a);
 }
 
 // Channel sets of the measured workloads get their own instantiations (registers sized for the

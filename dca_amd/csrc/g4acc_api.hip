@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -87,6 +88,12 @@ struct g4acc_handle {
   int mtog_mfma = 0;               // tuning: fused right M -> G kernel on the matrix cores
   int g4_qg = 0;                   // tuning: momenta per G4 workgroup (0: register limit)
   int g4_mfma = 0;                 // tuning: rank-one G4 terms on the matrix cores (FP32 G, Nc <= 16)
+  // tuning: rank-one MAGNETIC / CHARGE terms of a batch as one matrix-core GEMM (g4_rank1.hip),
+  // stored in d_R (complex64, one G4-sized plane per channel) and added by the tile kernel.
+  int g4_rank1 = 0;
+  void* d_R = nullptr;
+  int* d_r1_planes = nullptr;
+  int n_r1 = 0, r1_slot[4] = {-1, -1, -1, -1};
   std::vector<std::pair<int, hipGraphExec_t>> graphs;  // key: (walkers << 1) | stage
   // stage timing (profiling hook)
   bool timing = false;
@@ -379,6 +386,12 @@ void run_g4(g4acc_handle* h, int n_walkers) {
     a.tile_perm = h->d_tperm;
     a.sum_mode = h->sum_mode;
     a.mfma = h->g4_mfma;
+    const bool r1 = std::is_same<Real, float>::value && h->g4_rank1 && h->d_R;
+    a.R = r1 ? h->d_R : nullptr;
+    a.n_r1 = h->n_r1;
+    a.r1_planes = h->d_r1_planes;
+    for (int c = 0; c < 4; ++c)
+      a.r1_slot[c] = h->r1_slot[c];
     if (launch_g4_tiled<Real>(a, h->stream))
       return;
   }
@@ -1042,6 +1055,33 @@ int g4acc_set_tuning(g4acc_handle* h, const char* key, int value) {
     h->tile_order = value;
   else if (k == "g4_mfma" && (value == 0 || value == 1))
     h->g4_mfma = value;
+  else if (k == "g4_rank1" && (value == 0 || value == 1)) {
+    if (value && !h->d_R && h->use_tiled && h->precision != G4ACC_FP64) {
+      int rs;
+      if ((rs = h->check(hipSetDevice(h->device), "hipSetDevice")))
+        return rs;
+      // One complex64 plane of G4's (k1, w1, k2, w2, q, v) size per rank-one channel.
+      std::vector<int> planes;
+      for (size_t c = 0; c < h->channels.size(); ++c) {
+        const int ch = h->channels[c];
+        if (ch == G4ACC_PARTICLE_HOLE_MAGNETIC || ch == G4ACC_PARTICLE_HOLE_CHARGE) {
+          h->r1_slot[c] = static_cast<int>(planes.size());
+          planes.push_back(ch == G4ACC_PARTICLE_HOLE_MAGNETIC ? 0 : 1);  // U plane of uv_kernel
+        }
+      }
+      if (!planes.empty()) {
+        int rc;
+        if ((rc = h->alloc(reinterpret_cast<float2**>(&h->d_R), planes.size() * h->g4_entries, "R")) ||
+            (rc = h->alloc(&h->d_r1_planes, planes.size(), "r1_planes")))
+          return rc;
+        if ((rc = h->check(hipMemcpy(h->d_r1_planes, planes.data(), planes.size() * sizeof(int),
+                                     hipMemcpyHostToDevice), "hipMemcpy")))
+          return rc;
+        h->n_r1 = static_cast<int>(planes.size());
+      }
+    }
+    h->g4_rank1 = value;
+  }
   else if (k == "graph" && (value == 0 || value == 1))
     h->use_graph = value != 0;
   else

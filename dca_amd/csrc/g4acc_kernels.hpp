@@ -85,6 +85,13 @@ struct TiledArgs {
   const int* tile_perm;        // schedule 2: (w1 | w2 << 16) in dispatch order, nw * nw entries
   int sum_mode;
   int mfma;                    // 1: rank-one terms on the matrix cores where a kernel provides it
+  // Rank-one terms precomputed for the batch (complex64 G, 256-thread tiles): R[c][q,v][p2][p1] for
+  // the n_r1 channels with r1_planes[c] their U plane, and r1_slot[ch] the R plane of G4 channel
+  // slot ch (-1: none). R == nullptr: the tile kernel forms them itself.
+  void* R;
+  int n_r1;
+  const int* r1_planes;  // device, n_r1 entries
+  int r1_slot[4];
   bool need_y, need_z, has_trans;
 };
 
@@ -94,6 +101,8 @@ bool launch_g4_tiled(TiledArgs<Real> a, hipStream_t st);
 // false if it does not cover the geometry or channel set. The caller has run uv_kernel and set
 // a.nqg for momentum groups of qg.
 bool launch_g4_mfma(const TiledArgs<float>& a, int qg, hipStream_t st);
+// R of the batch from the rank-one factors (g4_rank1.hip); the caller has run uv_kernel.
+void launch_g4_rank1(const TiledArgs<float>& a, hipStream_t st);
 // Multiband LDS-tiled G4 update (g4_band.hip); false if the band count, cluster or channel set
 // is not covered (the generic kernel then runs).
 template <typename Real>

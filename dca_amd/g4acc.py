@@ -331,7 +331,8 @@ class TpAccumulator:
         return self.lib.g4acc_bytes(self._h)
 
     def set_tuning(self, **params):
-        """Launch-shape tuning: mtog_tw, mtog_mfma, g4_qg, g4_tile_order, g4_mfma, graph (include/g4acc.h)."""
+        """Launch-shape tuning: mtog_tw, mtog_mfma, g4_qg, g4_tile_order, g4_mfma, g4_rank1, graph
+        (include/g4acc.h)."""
         for k, v in params.items():
             self._check(self.lib.g4acc_set_tuning(self._h, k.encode(), int(v)))
 

@@ -175,6 +175,9 @@ int g4acc_set_timing(g4acc_handle* h, int enable);
  * 0 = as many as registers allow, 1, 2, 4), "g4_tile_order" (-1 by batch size, 0 natural,
  * 1 XCD-aware super-tiles, 2 XCD-aware diagonal strips with their mirror images), "g4_mfma"
  * (1: rank-one G4 terms on the matrix cores where available, FP32 G and Nc <= 16; 0: vector ALUs),
+ * "g4_rank1" (1: the batch's rank-one MAGNETIC / CHARGE terms as one matrix-core GEMM into a
+ * complex64 buffer of G4's size per such channel, allocated when the key is set; FP32 G,
+ * 256-thread tiles, measured channel sets; 0: formed per walker in the tile kernel),
  * "graph" (1: replay captured batch sequences, 0: launch eagerly).
  * Results do not depend on the tuning beyond FP32 summation order. */
 int g4acc_set_tuning(g4acc_handle* h, const char* key, int value);

@@ -304,6 +304,33 @@ def test_mfma_rank_one_matches_oracle(oracle_lib, name, channels):
         assert e_vec < 5e-7
 
 
+@pytest.mark.parametrize("name", ["1band_4x4_4q", "1band_8site", "1band_3x3_shift2"])
+@pytest.mark.parametrize("channels", [(MAG, CHG, PP), (MAG,), (CHG,), (MAG, PP), (CHG, PP)])
+def test_rank_one_gemm_matches_oracle(oracle_lib, name, channels):
+    """FP32 G4 with the batch's rank-one MAGNETIC / CHARGE terms from one matrix-core GEMM
+    (g4_rank1): against the FP64 oracle (1e-6) and against the tile kernel that forms them per
+    walker (5e-7). Nine walkers leave a partial step of four and a partial flush of eight."""
+    wl = case(name, channels=list(channels))
+    g = wl.geometry
+    ms = wl.draw(9)
+    ref = oracle_g4(oracle_lib, g, ms)
+    r1 = g4acc.TpAccumulator(g, precision="fp32", max_walkers=9, max_vertices=wl.n_max, tuned=False)
+    vec = g4acc.TpAccumulator(g, precision="fp32", max_walkers=9, max_vertices=wl.n_max, tuned=False)
+    r1.set_tuning(g4_rank1=1)
+    vec.set_tuning(g4_rank1=0)
+    for acc in (r1, vec):
+        acc.resetAccumulation()
+        acc.accumulate_batch(ms)
+        acc.accumulate_batch(ms[:5])  # a second, smaller batch reuses the R planes
+    ref5 = oracle_g4(oracle_lib, g, ms[:5])
+    for ch, r, r5, a, b in zip(g.channels, ref, ref5, r1.get_G4(), vec.get_G4()):
+        e_ref, e_vec = linf(r + r5, a), linf(b, a)
+        print(f"{name} channel {ch}: rank-one GEMM vs oracle {e_ref:.3e}, vs tile kernel {e_vec:.3e}",
+              flush=True)
+        assert e_ref < 1e-6
+        assert e_vec < 5e-7
+
+
 @pytest.mark.parametrize("name", ["1band_2x2", "1band_4x4_4q", "1band_6x6", "1band_8site"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_fused_equals_unfused_mtog(name, precision, monkeypatch):
