@@ -172,6 +172,9 @@ def main():
     bytes_g4_stage = n_ch * 32 * g4_entries + W * 2 * g.D * g.D * 8
     achieved = bytes_g4_stage / (g4_ms * 1e-3) / 1e9
     stage = "g4" if g4_ms >= mtog_ms else "mtog"
+    r1 = bool((mixed.tuning or {}).get("g4_rank1"))
+    g4_kernels = ("G4 update (uv_kernel" + (" + g4_rank1_kernel" if r1 else "") +
+                  " + g4_tiled_kernel, one launch each per batch)")
     traffic, traffic_src = measured_traffic(args.config, W)
 
     # ---- accuracy against the FP64 CPU oracle, and the CPU baseline ---------------------------
@@ -225,7 +228,7 @@ def main():
                               "plain_mtog": round(p_mtog, 4), "plain_g4": round(p_g4, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "G4 update (uv_kernel + g4_tiled_kernel, one launch each per batch)",
+                     "kernel": g4_kernels,
                      "dominant_stage": stage,
                      "algorithmic_bytes_per_launch": bytes_g4_stage,
                      "traffic_source": traffic_src},

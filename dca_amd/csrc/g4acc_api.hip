@@ -1055,6 +1055,15 @@ int g4acc_set_tuning(g4acc_handle* h, const char* key, int value) {
     h->tile_order = value;
   else if (k == "g4_mfma" && (value == 0 || value == 1))
     h->g4_mfma = value;
+  else if (k == "g4_tile_strip" && value >= 1 && value <= 16) {
+    // Diagonal positions per group of schedule 2 (table uploaded now, outside any batch).
+    const std::vector<int> tperm = diagonal_tile_order(h->g.nw, value);
+    int rs;
+    if ((rs = h->check(hipSetDevice(h->device), "hipSetDevice")) ||
+        (rs = h->check(hipMemcpy(h->d_tperm, tperm.data(), tperm.size() * sizeof(int), hipMemcpyHostToDevice),
+                       "hipMemcpy")))
+      return rs;
+  }
   else if (k == "g4_rank1" && (value == 0 || value == 1)) {
     if (value && !h->d_R && h->use_tiled && h->precision != G4ACC_FP64) {
       int rs;

@@ -173,7 +173,8 @@ int g4acc_set_timing(g4acc_handle* h, int enable);
  * matrix cores where available, FP32 G, Nc <= 8 or a 4x4 torus, max_vertices <= 4 (128 - Nc);
  * 0: vector ALUs), "g4_qg" (momenta per G4 workgroup:
  * 0 = as many as registers allow, 1, 2, 4), "g4_tile_order" (-1 by batch size, 0 natural,
- * 1 XCD-aware super-tiles, 2 XCD-aware diagonal strips with their mirror images), "g4_mfma"
+ * 1 XCD-aware super-tiles, 2 XCD-aware diagonal strips with their mirror images),
+ * "g4_tile_strip" (diagonal positions per group of schedule 2: 1..16, default 2), "g4_mfma"
  * (1: rank-one G4 terms on the matrix cores where available, FP32 G and Nc <= 16; 0: vector ALUs),
  * "g4_rank1" (1: the batch's rank-one MAGNETIC / CHARGE terms as one matrix-core GEMM into a
  * complex64 buffer of G4's size per such channel, allocated when the key is set; FP32 G,
