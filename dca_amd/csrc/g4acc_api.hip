@@ -382,7 +382,7 @@ void run_g4(g4acc_handle* h, int n_walkers) {
     a.qg_cap = h->g4_qg;
     // XCD-aware super-tiles pay off when many walkers re-read G; a single walker's launch is
     // dominated by the G4 read-modify-write, which the natural order streams best.
-    a.tile_order = h->tile_order >= 0 ? h->tile_order : (n_walkers > 1 ? 1 : 0);
+    a.tile_order = h->tile_order >= 0 ? h->tile_order : (n_walkers > 1 ? 2 : 0);
     a.tile_perm = h->d_tperm;
     a.sum_mode = h->sum_mode;
     a.mfma = h->g4_mfma;
@@ -719,6 +719,19 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
     launch_zero16(p, h->g4_entries, h->stream);
   if ((rc = h->check(hipStreamSynchronize(h->stream), "hipStreamSynchronize")))
     return bail(rc);
+  // Default launch shapes for callers that do not tune (the C++ drop-in): the matrix-core forms
+  // where they were measured faster, complex64 G on 16-site clusters with batches of 8 or more.
+  if (h->precision != G4ACC_FP64 && g.nb == 1 && g.nc == 16) {
+    h->mtog_mfma = 1;
+    int mask = 0;
+    for (int c : h->channels)
+      mask |= 1 << c;
+    constexpr int M = 1 << G4ACC_PARTICLE_HOLE_MAGNETIC, Q = 1 << G4ACC_PARTICLE_HOLE_CHARGE,
+                  P = 1 << G4ACC_PARTICLE_PARTICLE_UP_DOWN;
+    if (h->use_tiled && h->max_walkers >= 8 && (mask == (M | Q | P) || mask == (M | P) || mask == M || mask == Q) &&
+        (rc = g4acc_set_tuning(h, "g4_rank1", 1)) != G4ACC_OK)
+      return bail(rc);
+  }
   *out = h;
   return G4ACC_OK;
 }

@@ -172,7 +172,8 @@ int g4acc_set_timing(g4acc_handle* h, int enable);
  * fused single-band M -> G kernel: 0 default, 1, 2, 4, 8), "mtog_mfma" (1: its right half on the
  * matrix cores where available, FP32 G, Nc <= 8 or a 4x4 torus, max_vertices <= 4 (128 - Nc);
  * 0: vector ALUs), "g4_qg" (momenta per G4 workgroup:
- * 0 = as many as registers allow, 1, 2, 4), "g4_tile_order" (-1 by batch size, 0 natural,
+ * 0 = as many as registers allow, 1, 2, 4), "g4_tile_order" (-1 by batch size: 2 for batches, 0 for
+ * one walker; 0 natural,
  * 1 XCD-aware super-tiles, 2 XCD-aware diagonal strips with their mirror images),
  * "g4_tile_strip" (diagonal positions per group of schedule 2: 1..16, default 2), "g4_mfma"
  * (1: rank-one G4 terms on the matrix cores where available, FP32 G and Nc <= 16; 0: vector ALUs),
@@ -180,6 +181,8 @@ int g4acc_set_timing(g4acc_handle* h, int enable);
  * complex64 buffer of G4's size per such channel, allocated when the key is set; FP32 G,
  * 256-thread tiles, measured channel sets; 0: formed per walker in the tile kernel),
  * "graph" (1: replay captured batch sequences, 0: launch eagerly).
+ * Defaults: mtog_mfma = 1 and, with max_walkers >= 8 and a measured channel set, g4_rank1 = 1 for
+ * complex64 G on single-band 16-site clusters; 0 elsewhere.
  * Results do not depend on the tuning beyond FP32 summation order. */
 int g4acc_set_tuning(g4acc_handle* h, const char* key, int value);
 int g4acc_get_timing(const g4acc_handle* h, double* mtog_ms, double* g4_ms, long long* batches);

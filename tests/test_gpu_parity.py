@@ -292,7 +292,7 @@ def test_mfma_rank_one_matches_oracle(oracle_lib, name, channels):
     mf = g4acc.TpAccumulator(g, precision="fp32", max_walkers=9, max_vertices=wl.n_max, tuned=False)
     vec = g4acc.TpAccumulator(g, precision="fp32", max_walkers=9, max_vertices=wl.n_max, tuned=False)
     mf.set_tuning(g4_mfma=1)
-    vec.set_tuning(g4_mfma=0)
+    vec.set_tuning(g4_mfma=0, g4_rank1=0)
     for acc in (mf, vec):
         acc.resetAccumulation()
         acc.accumulate_batch(ms)
