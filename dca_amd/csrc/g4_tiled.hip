@@ -372,7 +372,7 @@ namespace {
 template <typename Real, int TT, int PPT, int QG, int NCH, int MASK>
 void launch_tiled_fixed(TiledArgs<Real> a, size_t lds, hipStream_t st) {
   const unsigned tiles = tile_schedule(a);
-  constexpr bool kRX = std::is_same<Real, float>::value && TT == 256 && MASK != 0 &&
+  constexpr bool kRX = std::is_same<Real, float>::value && MASK != 0 &&
                        (MASK & ((1 << PH_MAGNETIC) | (1 << PH_CHARGE))) != 0;
   if constexpr (kRX) {
     if (a.R) {
@@ -485,8 +485,15 @@ bool launch_g4_tiled(TiledArgs<Real> a, hipStream_t st) {
     qg = 2;
   if (ppt > 2)
     qg = 1;
-  // FP64 accumulators per thread: 16 for 256-thread workgroups, 4 for 1024-thread ones.
-  const int acc_budget = ts.threads == 256 ? 16 : 4;
+  // FP64 accumulators per thread: 16 for 256-thread workgroups, 4 for 1024-thread ones (8 when
+  // the batch's rank-one terms come precomputed, which frees the factor staging and partials).
+  int mask = 0;
+  for (int c = 0; c < a.n_ch; ++c)
+    mask |= 1 << a.ch[c];
+  constexpr int kM = 1 << PH_MAGNETIC, kQ = 1 << PH_CHARGE, kP = 1 << PP_UP_DOWN;
+  const bool rx = std::is_same<Real, float>::value && a.R != nullptr &&
+                  (mask == (kM | kQ | kP) || mask == (kM | kP) || mask == kQ || mask == kM);
+  const int acc_budget = ts.threads == 256 ? 16 : (rx ? 8 : 4);
   while (qg > 1 && ppt * qg * a.n_ch > acc_budget)
     qg /= 2;
   while (a.qg_cap > 0 && qg > a.qg_cap)

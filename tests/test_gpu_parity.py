@@ -304,12 +304,13 @@ def test_mfma_rank_one_matches_oracle(oracle_lib, name, channels):
         assert e_vec < 5e-7
 
 
-@pytest.mark.parametrize("name", ["1band_4x4_4q", "1band_8site", "1band_3x3_shift2"])
+@pytest.mark.parametrize("name", ["1band_4x4_4q", "1band_8site", "1band_3x3_shift2", "1band_6x6"])
 @pytest.mark.parametrize("channels", [(MAG, CHG, PP), (MAG,), (CHG,), (MAG, PP), (CHG, PP)])
 def test_rank_one_gemm_matches_oracle(oracle_lib, name, channels):
     """FP32 G4 with the batch's rank-one MAGNETIC / CHARGE terms from one matrix-core GEMM
     (g4_rank1): against the FP64 oracle (1e-6) and against the tile kernel that forms them per
-    walker (5e-7). Nine walkers leave a partial step of four and a partial flush of eight."""
+    walker (5e-7). Nine walkers leave a partial step of four and a partial flush of eight; the
+    6x6 cluster runs the 1024-thread tile (two pairs per thread, two momenta per workgroup)."""
     wl = case(name, channels=list(channels))
     g = wl.geometry
     ms = wl.draw(9)
