@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "G4 measurements/s: batched MFMA-FP32 vs plain FP64 at max rel err <=1e-6"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector and matrix)
 
 
 def parse():
@@ -50,15 +51,20 @@ def log(msg):
 
 
 def measured_traffic(cfg, walkers):
-    """HBM bytes per launch of the G4 kernel from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the same workload, or None."""
+    """HBM bytes per launch of the G4 stage from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the same workload and batch size, or None."""
     path = os.path.join(ROOT, "profiles", f"g4_update_cfg{cfg}_hbm_traffic.txt")
-    if cfg != 2 or walkers != 64 or not os.path.exists(path):
+    if not os.path.exists(path):
         return None, None
+    traffic, w = None, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":")[1]), os.path.relpath(path, ROOT)
-    return None, None
+            traffic = int(line.split(":")[1])
+        if line.startswith("walkers_per_launch:"):
+            w = int(line.split(":")[1])
+    if traffic is None or w != walkers:
+        return None, None
+    return traffic, os.path.relpath(path, ROOT)
 
 
 def run_batches(acc, batches, steps):
@@ -97,10 +103,12 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(measurements shard across GPUs by walker, see DESIGN.md)"}))
+                                   "(measurements shard across GPUs by walker: one process per GPU, "
+                                   "G4 merged with one sum-reduce at the end)"}))
         sys.exit(2)
     import torch
     from dca_amd import g4acc, synth
+    from dca_amd import geometry as geo
     from oracle import bruteforce as bf
 
     if args.copy_calibration:
@@ -171,6 +179,14 @@ def main():
     g4_entries = g.g4_size()
     bytes_g4_stage = n_ch * 32 * g4_entries + W * 2 * g.D * g.D * 8
     achieved = bytes_g4_stage / (g4_ms * 1e-3) / 1e9
+    # Algorithmic FP32 work of the same launch: per walker and G4 entry (k1, w1, k2, w2, q, v) the
+    # complex multiply-adds of the single-band channels (tp_accumulator_cpu.hpp:491-661): rank-one
+    # MAGNETIC and CHARGE, the crossed sum X0 Y0 + X1 Y1 shared by both, X0 Z1 + X1 Z0 for PP
+    # (8 flops each). With batches the stage stops being HBM-bound; this is its compute rate.
+    cfma = {geo.PARTICLE_HOLE_MAGNETIC: 1, geo.PARTICLE_HOLE_CHARGE: 1, geo.PARTICLE_PARTICLE_UP_DOWN: 2}
+    n_cfma = sum(cfma.get(c, 0) for c in g.channels) + (
+        2 if any(c in (geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE) for c in g.channels) else 0)
+    flops_g4_stage = 8 * n_cfma * g4_entries * W if g.nb == 1 else None
     stage = "g4" if g4_ms >= mtog_ms else "mtog"
     r1 = bool((mixed.tuning or {}).get("g4_rank1"))
     g4_kernels = ("G4 update (uv_kernel" + (" + g4_rank1_kernel" if r1 else "") +
@@ -231,7 +247,12 @@ def main():
                      "kernel": g4_kernels,
                      "dominant_stage": stage,
                      "algorithmic_bytes_per_launch": bytes_g4_stage,
-                     "traffic_source": traffic_src},
+                     "traffic_source": traffic_src,
+                     "compute": None if flops_g4_stage is None else {
+                         "algorithmic_flops_per_launch": flops_g4_stage,
+                         "achieved_tflops": round(flops_g4_stage / (g4_ms * 1e-3) / 1e12, 2),
+                         "fp32_peak_tflops": FP32_PEAK_TFLOPS,
+                         "frac": round(flops_g4_stage / (g4_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

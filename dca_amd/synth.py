@@ -11,7 +11,7 @@ Box-Muller, Poisson by inversion.
 The five workloads are the accumulator configurations this project measures:
   1  1 band, 2x2 cluster, beta 2, 16 frequencies, w_ex {0}, k_ex {(0,0)}, magnetic, N = 32
   2  1 band, 4x4 cluster, beta 10, 64 frequencies, w_ex {0,1,2}, 4 k_ex, three channels,
-     N ~ Poisson(128) clipped to [64, 256]
+     N ~ Poisson(128) clipped to [64, 256], 256 walkers per batch
   3  1 band, 6x6 cluster, beta 20, 128 frequencies, w_ex {0,1}, 4 k_ex, magnetic + pp,
      N ~ U[50, 600], 64 walkers per batch
   4  3 bands, 8-site cluster (2,2),(2,-2), beta 15, 64 frequencies, w_ex {0,1}, 2 k_ex,
@@ -140,7 +140,7 @@ def workload(cfg: int) -> Workload:
         g = geo.make_geometry(1, cl, 10.0, 64, geo.exchange_frequencies(2, True), kex,
                               [geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE,
                                geo.PARTICLE_PARTICLE_UP_DOWN])
-        return Workload(2, g, _poisson_clipped(128, 64, 256), True, 64, 256, 10000,
+        return Workload(2, g, _poisson_clipped(128, 64, 256), True, 256, 256, 10000,
                         "1 band, 4x4 cluster, beta=10, 64 freqs, 4 k_ex, 3 w_ex, 3 channels, "
                         "N~Poisson(128) in [64,256]")
     if cfg == 3:
