@@ -910,6 +910,10 @@ int g4acc_accumulate_queued(g4acc_handle* h, const g4acc_walker* walker, int mem
   return G4ACC_OK;
 }
 
+int g4acc_accumulate_single(g4acc_handle* h, const g4acc_walker* walker, int memory) {
+  return g4acc_accumulate_queued(h, walker, memory);
+}
+
 int g4acc_sync_copy(g4acc_handle* h) {
   if (!h)
     return G4ACC_INVALID_ARG;

@@ -244,6 +244,7 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
   const int k1 = p1 % NC, w1 = p1 / NC;
   const double2 ga = G0[s + 2 * (k1 + static_cast<size_t>(NC) * w1)];
   const double inv_nc = 1.0 / NC;
+  const double2 gneg{-ga.x * inv_nc, -ga.y * inv_nc};  // -G0(p1) / Nc, folded once per row
   C* Gs = G + static_cast<size_t>(slot) * g.D * g.D + p1;
   if constexpr (L > 0) {
     {
@@ -259,13 +260,9 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
         for (int kk = 0; kk < NC; ++kk) {
           const int k2 = Kl[kk];
           double2 m = to_d(e[tt][kk]);
-          m.x *= inv_nc;
-          m.y *= inv_nc;
           if (bphase)
             m = cmulc(cmul(m, bphase[k1]), bphase[k2]);
-          double2 v = cmul(cmul(ga, m), G0l[k2 + NC * (w2 - w2wg)]);
-          v.x = -v.x;
-          v.y = -v.y;
+          double2 v = cmul(cmul(gneg, m), G0l[k2 + NC * (w2 - w2wg)]);
           const int p2 = k2 + NC * w2;
           if (p1 == p2) {
             v.x = fma(ga.x, g.beta, v.x);
@@ -289,13 +286,9 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
       for (int r = 0; r < NC; ++r)
         cfmac(acc, e[tt][r], Dl[k2 + NC * r]);
       double2 m = to_d(acc);
-      m.x *= inv_nc;
-      m.y *= inv_nc;
       if (bphase)
         m = cmulc(cmul(m, bphase[k1]), bphase[k2]);
-      double2 v = cmul(cmul(ga, m), G0l[k2 + NC * (w2 - w2wg)]);
-      v.x = -v.x;
-      v.y = -v.y;
+      double2 v = cmul(cmul(gneg, m), G0l[k2 + NC * (w2 - w2wg)]);
       const int p2 = k2 + NC * w2;
       if (p1 == p2) {
         v.x = fma(ga.x, g.beta, v.x);
@@ -533,6 +526,7 @@ __global__ void __launch_bounds__(256) mtog_right_mfma(const SlotDesc* __restric
     const int k1 = p1 % NC, w1 = p1 / NC;
     const double2 ga = G0[s + 2 * (k1 + static_cast<size_t>(NC) * w1)];
     const double inv_nc = 1.0 / NC;
+    const double2 gneg{-ga.x * inv_nc, -ga.y * inv_nc};  // -G0(p1) / Nc, folded once per row
     C* Gs = G + static_cast<size_t>(slot) * g.D * g.D + p1;
 #pragma unroll
     for (int ii = 0; ii < NI; ++ii) {
@@ -563,13 +557,9 @@ __global__ void __launch_bounds__(256) mtog_right_mfma(const SlotDesc* __restric
       for (int kk = 0; kk < NC; ++kk) {
         const int k2 = L > 0 ? Kl[kk] : kk;
         double2 m = to_d(e[kk]);
-        m.x *= inv_nc;
-        m.y *= inv_nc;
         if (bphase)
           m = cmulc(cmul(m, bphase[k1]), bphase[k2]);
-        double2 v = cmul(cmul(ga, m), G0l[k2 + NC * (w2 - w0)]);
-        v.x = -v.x;
-        v.y = -v.y;
+        double2 v = cmul(cmul(gneg, m), G0l[k2 + NC * (w2 - w0)]);
         const int p2 = k2 + NC * w2;
         if (p1 == p2) {
           v.x = fma(ga.x, g.beta, v.x);

@@ -33,6 +33,7 @@ EXPORTED = [
     "g4acc_finalize", "g4acc_sync", "g4acc_get_g4", "g4acc_sum_to", "g4acc_g4_size", "g4acc_bytes",
     "g4acc_stream", "g4acc_last_error", "g4acc_compute_G", "g4acc_g_size", "g4acc_accumulate_G",
     "g4acc_set_timing", "g4acc_get_timing", "g4acc_set_tuning", "g4acc_accumulate_queued",
+    "g4acc_accumulate_single",
     "g4acc_flush", "g4acc_sync_copy",
 ]
 
@@ -118,6 +119,7 @@ def load_library(path: str = LIB_PATH):
         "g4acc_set_timing": (i32, [vp, i32]),
         "g4acc_set_tuning": (i32, [vp, ctypes.c_char_p, i32]),
         "g4acc_accumulate_queued": (i32, [vp, ctypes.POINTER(Walker_c), i32]),
+        "g4acc_accumulate_single": (i32, [vp, ctypes.POINTER(Walker_c), i32]),
         "g4acc_flush": (i32, [vp]),
         "g4acc_sync_copy": (i32, [vp]),
         "g4acc_get_timing": (i32, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),

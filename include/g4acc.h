@@ -127,6 +127,9 @@ int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_w
  * g4acc_sync, g4acc_finalize, g4acc_get_g4 and g4acc_sum_to; g4acc_reset discards it. Results
  * equal g4acc_accumulate_batch over the same measurements in the same chunks. */
 int g4acc_accumulate_queued(g4acc_handle* h, const g4acc_walker* walker, int memory);
+/* Same as g4acc_accumulate_queued: the one-measurement drop-in for accumulate(M, configuration,
+ * sign) (tp_accumulator_gpu.hpp:321-355) that batches internally. */
+int g4acc_accumulate_single(g4acc_handle* h, const g4acc_walker* walker, int memory);
 /* Runs the queued measurements now. */
 int g4acc_flush(g4acc_handle* h);
 /* synchronizeCopy() for queued measurements: returns once the caller's buffers have been copied

@@ -5,6 +5,7 @@ Tolerances (max-normalised l_inf of include/dca/function/util/difference.hpp:71-
   FP32 path (G, G4)    1e-6  (the accuracy gate of the mixed-precision form)
 The reference's own CPU-vs-GPU bar is 5e-7 (TT/tp/tp_accumulator_gpu_test.cpp:213).
 """
+import ctypes
 import math
 
 import numpy as np
@@ -302,6 +303,26 @@ def test_mfma_rank_one_matches_oracle(oracle_lib, name, channels):
               flush=True)
         assert e_ref < 1e-6
         assert e_vec < 5e-7
+
+
+def test_accumulate_single_is_the_queued_accumulate():
+    """g4acc_accumulate_single (the one-measurement drop-in name) batches like
+    g4acc_accumulate_queued: seven host measurements through a queue of three equal the batch."""
+    wl = case("1band_4x4_4q")
+    g = wl.geometry
+    ms = wl.draw(7)
+    bat = g4acc.TpAccumulator(g, precision="fp32", max_walkers=3, max_vertices=wl.n_max)
+    one = g4acc.TpAccumulator(g, precision="fp32", max_walkers=3, max_vertices=wl.n_max)
+    bat.resetAccumulation()
+    one.resetAccumulation()
+    for k in range(0, 7, 3):
+        bat.accumulate_batch(ms[k:k + 3])
+    keep = g4acc._Keep()
+    for meas in ms:
+        w = g4acc.walker_struct(meas, keep)
+        one._check(one.lib.g4acc_accumulate_single(one._h, ctypes.byref(w), g4acc.HOST))
+    for a, b in zip(bat.get_G4(), one.get_G4()):
+        assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("name", ["1band_4x4_4q", "1band_8site", "1band_3x3_shift2", "1band_6x6"])
