@@ -173,7 +173,9 @@ __global__ void __launch_bounds__(256) ndft_right_kernel(const SlotDesc* __restr
 
 // S4: right site DFT, band phases and G = -G0 M~ G0 + beta G0 delta for one (k1,w1,k2,w2) band
 // block per thread; stores the band-transposed block (see the file comment).
-template <typename Real>
+// NB bands as a compile-time constant: the nb x nb blocks M~(e, f) and the G0 factors stay in
+// registers (with a run-time band count they went to scratch memory).
+template <typename Real, int NB>
 __global__ void __launch_bounds__(256) dft_right_G_kernel(Geo g, int slot0,
                                                           const Cplx<Real>* __restrict__ dft,
                                                           const double2* __restrict__ bphase,
@@ -181,7 +183,6 @@ __global__ void __launch_bounds__(256) dft_right_G_kernel(Geo g, int slot0,
                                                           const Cplx<Real>* __restrict__ E,
                                                           Cplx<Real>* __restrict__ G) {
   using C = Cplx<Real>;
-  constexpr int MAXB = 4;
   const int slot = slot0 + blockIdx.z;
   const int s = slot & 1;
   const int KW = g.nc * g.nwE;
@@ -191,49 +192,65 @@ __global__ void __launch_bounds__(256) dft_right_G_kernel(Geo g, int slot0,
     return;
   const int k1 = kw1 % g.nc, w1 = kw1 / g.nc;
   const int k2 = kw2 % g.nc, w2 = kw2 / g.nc;
-  const int nb = g.nb;
   const C* Es = E + static_cast<size_t>(blockIdx.z) * g.D * g.D;
   const double inv_nc = 1.0 / g.nc;
-  double2 mt[MAXB][MAXB];
-  for (int e = 0; e < nb; ++e) {
-    const size_t row = e + static_cast<size_t>(nb) * kw1;
-    for (int f = 0; f < nb; ++f) {
+  double2 mt[NB][NB];
+#pragma unroll
+  for (int e = 0; e < NB; ++e) {
+    const size_t row = e + static_cast<size_t>(NB) * kw1;
+#pragma unroll
+    for (int f = 0; f < NB; ++f) {
       C acc{0, 0};
       for (int r2 = 0; r2 < g.nc; ++r2)
-        cfmac(acc, Es[row + static_cast<size_t>(g.D) * (f + nb * r2 + static_cast<size_t>(g.n_orb) * w2)],
+        cfmac(acc, Es[row + static_cast<size_t>(g.D) * (f + NB * r2 + static_cast<size_t>(g.n_orb) * w2)],
               dft[k2 + g.nc * r2]);
       double2 v = to_d(acc);
       v.x *= inv_nc;
       v.y *= inv_nc;
       if (bphase)
-        v = cmulc(cmul(v, bphase[e + nb * k1]), bphase[f + nb * k2]);
+        v = cmulc(cmul(v, bphase[e + NB * k1]), bphase[f + NB * k2]);
       mt[e][f] = v;
     }
   }
-  // G0 index (b1, b2, s, k, w)
+  // G0 index (b1, b2, s, k, w): the left factors G0(c, e; k1, w1) per thread, the right ones
+  // G0(f, dd; k2, w2) shared by the workgroup.
   auto g0 = [&](int b1, int b2, int k, int w) {
-    return G0[b1 + nb * (b2 + nb * (s + 2 * (k + static_cast<size_t>(g.nc) * w)))];
+    return G0[b1 + NB * (b2 + NB * (s + 2 * (k + static_cast<size_t>(g.nc) * w)))];
   };
+  double2 gl[NB][NB], tmp[NB][NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int e = 0; e < NB; ++e)
+      gl[c][e] = g0(c, e, k1, w1);
+#pragma unroll
+  for (int e = 0; e < NB; ++e)
+#pragma unroll
+    for (int dd = 0; dd < NB; ++dd) {
+      double2 t{0, 0};
+#pragma unroll
+      for (int f = 0; f < NB; ++f)
+        cfma(t, mt[e][f], g0(f, dd, k2, w2));
+      tmp[e][dd] = t;
+    }
   C* Gs = G + static_cast<size_t>(slot) * g.D * g.D;
   const bool diag = (k1 == k2) && (w1 == w2);
-  for (int c = 0; c < nb; ++c)
-    for (int dd = 0; dd < nb; ++dd) {
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int dd = 0; dd < NB; ++dd) {
       double2 acc{0, 0};
-      for (int e = 0; e < nb; ++e) {
-        double2 tmp{0, 0};
-        for (int f = 0; f < nb; ++f)
-          cfma(tmp, mt[e][f], g0(f, dd, k2, w2));
-        cfma(acc, g0(c, e, k1, w1), tmp);
-      }
+#pragma unroll
+      for (int e = 0; e < NB; ++e)
+        cfma(acc, gl[c][e], tmp[e][dd]);
       acc.x = -acc.x;
       acc.y = -acc.y;
       if (diag) {
-        const double2 z = g0(c, dd, k1, w1);
-        acc.x = fma(z.x, g.beta, acc.x);
-        acc.y = fma(z.y, g.beta, acc.y);
+        acc.x = fma(gl[c][dd].x, g.beta, acc.x);
+        acc.y = fma(gl[c][dd].y, g.beta, acc.y);
       }
       // value G_(c, dd, s, k1, k2, w1, w2) is stored at row (dd, k1, w1), column (c, k2, w2)
-      Gs[dd + static_cast<size_t>(nb) * kw1 + static_cast<size_t>(g.D) * (c + static_cast<size_t>(nb) * kw2)] =
+      Gs[dd + static_cast<size_t>(NB) * kw1 + static_cast<size_t>(g.D) * (c + static_cast<size_t>(NB) * kw2)] =
           from_d<C>(acc);
     }
 }
@@ -264,8 +281,13 @@ void launch_mtog(const MtoGArgs<Real>& a, hipStream_t st) {
     hipLaunchKernelGGL(ndft_right_kernel<Real>, dim3((g.D + 255) / 256, g.D, ns), dim3(256), 0, st,
                        a.desc, g, a.n_max, s0, a.off, a.T, a.A, a.E);
     const int KW = g.nc * g.nwE;
-    hipLaunchKernelGGL(dft_right_G_kernel<Real>, dim3((KW + 255) / 256, KW, ns), dim3(256), 0, st,
-                       g, s0, a.dft, a.bphase, a.G0, a.E, a.G);
+    const dim3 grid((KW + 255) / 256, KW, ns);
+    switch (g.nb) {
+      case 1: hipLaunchKernelGGL((dft_right_G_kernel<Real, 1>), grid, dim3(256), 0, st, g, s0, a.dft, a.bphase, a.G0, a.E, a.G); break;
+      case 2: hipLaunchKernelGGL((dft_right_G_kernel<Real, 2>), grid, dim3(256), 0, st, g, s0, a.dft, a.bphase, a.G0, a.E, a.G); break;
+      case 3: hipLaunchKernelGGL((dft_right_G_kernel<Real, 3>), grid, dim3(256), 0, st, g, s0, a.dft, a.bphase, a.G0, a.E, a.G); break;
+      default: hipLaunchKernelGGL((dft_right_G_kernel<Real, 4>), grid, dim3(256), 0, st, g, s0, a.dft, a.bphase, a.G0, a.E, a.G); break;
+    }
   }
 }
 
