@@ -31,6 +31,19 @@ struct alignas(2 * sizeof(C)) CPair {
   C a, b;
 };
 
+// A 16-byte pair is stored whole (one ds_write_b128); a 32-byte one member by member, since a
+// 32-byte aligned temporary goes through scratch memory.
+template <typename C>
+__device__ __forceinline__ void store_pair(CPair<C>& dst, C a, C b) {
+  if constexpr (sizeof(C) == 8) {
+    dst = CPair<C>{a, b};
+  }
+  else {
+    dst.a = a;
+    dst.b = b;
+  }
+}
+
 // UV[((m * 4 + plane) * nqv + qv) * P + p], planes: a(u0-u1), a(u0+u1), v0-v1, v0+v1.
 template <typename Real>
 __global__ void __launch_bounds__(256) uv_kernel(TiledArgs<Real> a) {
@@ -205,9 +218,9 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
       if (idx < nc * nc) {
         const int li = (idx / nc) * ld + (idx % nc);
         if (needY)
-          Yl[li] = P2{pre[0][j], pre[1][j]};
+          store_pair(Yl[li], pre[0][j], pre[1][j]);
         if (needZ)
-          Zl[li] = P2{pre[2][j], pre[3][j]};
+          store_pair(Zl[li], pre[2][j], pre[3][j]);
       }
     }
     if (u_valid)
@@ -276,8 +289,7 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
         else {
           C cross{0, 0}, ppz{0, 0}, trans{0, 0};
           if (needY) {
-            const P2 yy = Yl[yi[j][qi]];
-            const C y0 = yy.a, y1 = yy.b;
+            const C y0 = Yl[yi[j][qi]].a, y1 = Yl[yi[j][qi]].b;  // members: no 32-byte temporary
             cross = cmul(x0, y0);
             cfma(cross, x1, y1);
             if (has_trans) {
@@ -286,9 +298,8 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
             }
           }
           if (needZ) {
-            const P2 zz = Zl[zi[j][qi]];
-            ppz = cmul(x0, zz.b);
-            cfma(ppz, x1, zz.a);
+            ppz = cmul(x0, Zl[zi[j][qi]].b);
+            cfma(ppz, x1, Zl[zi[j][qi]].a);
           }
 #pragma unroll
           for (int c = 0; c < NCH; ++c) {
