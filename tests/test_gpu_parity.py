@@ -253,25 +253,26 @@ def test_tiled_equals_generic_kernel(name, precision, monkeypatch):
 @pytest.mark.parametrize("name", ["1band_4x4_4q", "1band_6x6", "1band_8site"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_tile_schedules_agree(oracle_lib, name, precision):
-    """The three G4 tile schedules (natural, super-tiles, diagonal strips) cover every tile once:
-    same G4 up to summation order, and the oracle's."""
+    """The G4 tile schedules (natural, super-tiles, diagonal strips of several widths) cover every
+    tile once: same G4 up to summation order, and the oracle's."""
     wl = case(name)
     g = wl.geometry
     ms = wl.draw(5)
     ref = oracle_g4(oracle_lib, g, ms)
     out = []
-    for order in (0, 1, 2):
+    # (order, strip): natural, super-tiles, diagonal strips of 2 (default), 1 and 5 positions
+    for order, strip in ((0, 2), (1, 2), (2, 2), (2, 1), (2, 5)):
         acc = g4acc.TpAccumulator(g, precision=precision, max_walkers=5, max_vertices=wl.n_max,
                                   tuned=False)
-        acc.set_tuning(g4_tile_order=order)
+        acc.set_tuning(g4_tile_order=order, g4_tile_strip=strip)
         acc.resetAccumulation()
         acc.accumulate_batch(ms)
         out.append(acc.get_G4())
         acc.close()
     for c, r in enumerate(ref):
-        for order in (1, 2):
-            err = linf(out[0][c], out[order][c])
-            print(f"{name} {precision} order {order} vs natural l_inf={err:.3e}", flush=True)
+        for k in range(1, len(out)):
+            err = linf(out[0][c], out[k][c])
+            print(f"{name} {precision} schedule {k} vs natural l_inf={err:.3e}", flush=True)
             assert err < (1e-14 if precision == "fp64" else 1e-7)
         assert linf(r, out[2][c]) < (1e-12 if precision == "fp64" else 1e-6)
 
