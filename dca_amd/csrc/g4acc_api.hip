@@ -960,6 +960,19 @@ int g4acc_get_g4(g4acc_handle* h, int channel_index, double* out, size_t n_compl
   return rc;
 }
 
+int g4acc_get_g4_normalized(g4acc_handle* h, int channel_index, double accumulated_sign, double* out,
+                            size_t n_complex) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  if (accumulated_sign == 0.0)
+    return h->fail(G4ACC_INVALID_ARG, "accumulated sign is zero");
+  G4_TRY(g4acc_get_g4(h, channel_index, out, n_complex));
+  const double f = 1.0 / (h->g.beta * h->g.beta * accumulated_sign);
+  for (size_t i = 0; i < 2 * h->g4_entries; ++i)
+    out[i] *= f;
+  return G4ACC_OK;
+}
+
 int g4acc_sum_to(g4acc_handle* src, g4acc_handle* dst) {
   if (!src || !dst || src == dst)
     return G4ACC_INVALID_ARG;

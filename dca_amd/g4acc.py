@@ -30,7 +30,7 @@ HOST, DEVICE_M, DEVICE = 0, 1, 3
 
 EXPORTED = [
     "g4acc_create", "g4acc_destroy", "g4acc_reset", "g4acc_accumulate", "g4acc_accumulate_batch",
-    "g4acc_finalize", "g4acc_sync", "g4acc_get_g4", "g4acc_sum_to", "g4acc_g4_size", "g4acc_bytes",
+    "g4acc_finalize", "g4acc_sync", "g4acc_get_g4", "g4acc_get_g4_normalized", "g4acc_sum_to", "g4acc_g4_size", "g4acc_bytes",
     "g4acc_stream", "g4acc_last_error", "g4acc_compute_G", "g4acc_g_size", "g4acc_accumulate_G",
     "g4acc_set_timing", "g4acc_get_timing", "g4acc_set_tuning", "g4acc_accumulate_queued",
     "g4acc_accumulate_single",
@@ -108,6 +108,7 @@ def load_library(path: str = LIB_PATH):
         "g4acc_finalize": (i32, [vp]),
         "g4acc_sync": (i32, [vp]),
         "g4acc_get_g4": (i32, [vp, i32, vp, sz]),
+        "g4acc_get_g4_normalized": (i32, [vp, i32, ctypes.c_double, vp, sz]),
         "g4acc_sum_to": (i32, [vp, vp]),
         "g4acc_g4_size": (sz, [vp]),
         "g4acc_bytes": (sz, [vp]),
@@ -302,6 +303,19 @@ class TpAccumulator:
         for c in range(len(self.geom.channels)):
             out = np.empty(n, np.complex128)
             self._check(self.lib.g4acc_get_g4(self._h, c, out.ctypes.data, n))
+            res.append(out)
+        self._inflight.clear()
+        return res
+
+    def get_G4_normalized(self, accumulated_sign: float) -> list[np.ndarray]:
+        """get_G4 divided by beta^2 * accumulated_sign, the solvers' normalisation
+        (ctaux_cluster_solver.hpp:523-527); accumulated_sign is the sum of the measurement signs."""
+        n = self.lib.g4acc_g4_size(self._h)
+        res = []
+        for c in range(len(self.geom.channels)):
+            out = np.empty(n, np.complex128)
+            self._check(self.lib.g4acc_get_g4_normalized(self._h, c, float(accumulated_sign),
+                                                         out.ctypes.data, n))
             res.append(out)
         self._inflight.clear()
         return res

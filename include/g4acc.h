@@ -142,6 +142,11 @@ int g4acc_sync(g4acc_handle* h);
 /* get_G4()[channel_index] (tp_accumulator_cpu.hpp:871-877): copies n_complex = g4acc_g4_size()
  * complex doubles in the reference order. */
 int g4acc_get_g4(g4acc_handle* h, int channel_index, double* out, size_t n_complex);
+/* The solvers' G4 normalisation after the accumulation (ctaux_cluster_solver.hpp:523-527,
+ * ctint_cluster_solver.hpp:511-516): get_g4 divided by beta^2 * accumulated_sign, in FP64.
+ * accumulated_sign is the solver's sum of the measurement signs (0: G4ACC_INVALID_ARG). */
+int g4acc_get_g4_normalized(g4acc_handle* h, int channel_index, double accumulated_sign, double* out,
+                            size_t n_complex);
 /* sumTo(other) (tp_accumulator_cpu.hpp:895-904): dst.G4 += src.G4 on the device, then src.G4
  * is cleared (get_g4 on src returns G4ACC_EMPTY until the next g4acc_reset). */
 int g4acc_sum_to(g4acc_handle* src, g4acc_handle* dst);

@@ -155,6 +155,17 @@ public:
     return G4_;
   }
 
+  // The solvers' normalisation of G4 (ctaux_cluster_solver.hpp:523-527): get_G4() divided by
+  // beta^2 * accumulated_sign, in FP64.
+  std::vector<TpGreensFunction> get_G4_normalized(double accumulated_sign) {
+    const std::size_t n = g4acc_g4_size(handle_);
+    std::vector<TpGreensFunction> out(geometry_.channels.size(), TpGreensFunction(n));
+    for (std::size_t c = 0; c < out.size(); ++c)
+      check(g4acc_get_g4_normalized(handle_, static_cast<int>(c), accumulated_sign,
+                                    reinterpret_cast<double*>(out[c].data()), n));
+    return out;
+  }
+
   // sumTo(other) (tp_accumulator_cpu.hpp:895-904): adds into other and empties this G4.
   void sumTo(TpAccumulator& other) { check(g4acc_sum_to(handle_, other.handle_)); }
 

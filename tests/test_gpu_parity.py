@@ -306,6 +306,22 @@ def test_mfma_rank_one_matches_oracle(oracle_lib, name, channels):
         assert e_vec < 5e-7
 
 
+def test_normalized_g4_is_g4_over_beta2_sign():
+    """The solvers' normalisation G4 / (beta^2 * accumulated sign) in FP64; a zero sign is refused."""
+    wl = case("1band_4x4_4q")
+    g = wl.geometry
+    ms = wl.draw(5)
+    acc = g4acc.TpAccumulator(g, precision="fp64", max_walkers=5, max_vertices=wl.n_max)
+    acc.resetAccumulation()
+    acc.accumulate_batch(ms)
+    sign_sum = sum(m.sign for m in ms)
+    raw = acc.get_G4()
+    for a, b in zip(raw, acc.get_G4_normalized(sign_sum)):
+        np.testing.assert_allclose(b, a / (g.beta ** 2 * sign_sum), rtol=1e-15, atol=0)
+    with pytest.raises(g4acc.LogicError):
+        acc.get_G4_normalized(0.0)
+
+
 def test_accumulate_single_is_the_queued_accumulate():
     """g4acc_accumulate_single (the one-measurement drop-in name) batches like
     g4acc_accumulate_queued: seven host measurements through a queue of three equal the batch."""
