@@ -88,6 +88,8 @@ def g4_free(geom: geo.Geometry, channel: int, sign: float) -> np.ndarray:
       MAG     : -sign beta^2 G0(b1,b3;k,w) G0(b4,b2;k+q,w+v)                    on k1=k2, w1=w2
       CHARGE  : -sign beta^2 G0(b4,b1;k,w) G0(b3,b2;k+q,w+v)                    on k1=k2, w1=w2
                 + 2 sign beta^2 G0(b1,b2;k1,w1) G0(b4,b3;k2,w2)                 if q = 0, v = 0
+      TRANSV. : -sign beta^2 G0(b4,b2;k,w) G0(b1,b3;k+q,w+v)                    on k1=k2, w1=w2
+    (TRANSVERSE from -a sum_s G_s(b2,b4;k1,k2) G_{-s}(b3,b1;k2+q,k1+q), tp_accumulator_cpu.hpp:454-489.)
     """
     nb, nc, nw = geom.nb, geom.nc, geom.n_freq
     ext = geom.ext
@@ -110,6 +112,9 @@ def g4_free(geom: geo.Geometry, channel: int, sign: float) -> np.ndarray:
             elif channel == geo.PARTICLE_HOLE_CHARGE:
                 g2 = g0[:, :, add[ks, q]][:, :, :, ws + nu + ext]
                 val = -sign * b2 * np.einsum("dakw,cbkw->abcdkw", g1, g2)
+            elif channel == geo.PARTICLE_HOLE_TRANSVERSE:
+                g2 = g0[:, :, add[ks, q]][:, :, :, ws + nu + ext]
+                val = -sign * b2 * np.einsum("dbkw,ackw->abcdkw", g1, g2)
             else:
                 raise ValueError("no closed form for this channel")
             for k in range(nc):

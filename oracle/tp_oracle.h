@@ -11,10 +11,10 @@
  *
  * Pins (tests/test_oracle.py): the NDFT against its defining double sum and a factorising rank-one
  * M, the single-vertex closed form, the direct M -> G definition, M = 0 closed forms of the
- * magnetic, charge and pp channels for one and three bands, linearity, sumTo, empty input.
- * Parity unpinned: the multiband band-slot placement of each channel's terms rests on the M = 0
- * closed forms, which are derived from the reference's source text (the reference's G4 baseline
- * files are missing blobs); PARTICLE_HOLE_TRANSVERSE has no closed-form check.
+ * transverse, magnetic, charge and pp channels for one and three bands, linearity, sumTo, empty
+ * input. Parity unpinned: the multiband band-slot placement of each channel's terms rests on the
+ * M = 0 closed forms, which are derived from the reference's source text (the reference's G4
+ * baseline files are missing blobs).
  *
  * Layouts (first index fastest, as dca::func::function stores them):
  *   G0   (b1, b2, s, k, wE)                 wE over the EXTENDED vertex frequencies

@@ -130,8 +130,6 @@ def test_free_closed_forms(oracle_lib, nb, L, sign):
     acc.accumulate(meas)
     g4 = acc.get_G4()
     for c, ch in enumerate(g.channels):
-        if ch == geo.PARTICLE_HOLE_TRANSVERSE:
-            continue  # out of scope; no closed form checked here
         ref = bf.g4_free(g, ch, sign).ravel(order="F")
         assert np.abs(ref).max() > 0
         assert bf.difference(ref, g4[c])["l_inf"] < 1e-14, geo.CHANNEL_NAMES[ch]
