@@ -168,28 +168,31 @@ def multiband_hamiltonian(k: np.ndarray, nb: int = 3, seed: int = 0x0DCA0004) ->
     return base[None] + 2.0 * c[:, None, None] * t[None]
 
 
-def free_g0(cluster: Cluster, nb: int, beta: float, n_ext: int) -> np.ndarray:
-    """Spin-independent non-interacting G0 over the EXTENDED vertex frequencies.
+def free_g0(cluster: Cluster, nb: int, beta: float, n_ext: int, zeeman: float = 0.0) -> np.ndarray:
+    """Non-interacting G0 over the EXTENDED vertex frequencies.
 
     Returns complex128 of shape (nb, nb, 2, nc, nwE) in C order (so that the Fortran-order
     flattening of its transpose matches the (b1, b2, s, k, wE) first-index-fastest layout; use
-    g0_flat() for that). Single band: g = 1/(i w - eps_k); several bands: (i w - H(k))^-1.
+    g0_flat() for that). Single band: g_s = 1/(i w - eps_k - h_s); several bands:
+    (i w - H(k) - h_s)^-1, with h_s = +zeeman for spin slot 0 and -zeeman for slot 1. zeeman = 0
+    (the measured workloads) gives a spin-independent G0; a non-zero field makes the two spin
+    sectors differ, so the spin-difference term of the magnetic channel and the s / -s pairing of
+    the transverse and pp channels no longer vanish or coincide.
     """
     w = vertex_frequencies(n_ext, beta)
     nc = cluster.nc
     out = np.zeros((nb, nb, 2, nc, n_ext), complex)
+    h = (float(zeeman), -float(zeeman))
     if nb == 1:
         eps = dispersion_single_band(cluster.k)
-        g = 1.0 / (1j * w[None, :] - eps[:, None])
-        out[0, 0, 0] = g
-        out[0, 0, 1] = g
+        for s in range(2):
+            out[0, 0, s] = 1.0 / (1j * w[None, :] - eps[:, None] - h[s])
         return out
     H = multiband_hamiltonian(cluster.k, nb)
-    for ik in range(nc):
-        for iw in range(n_ext):
-            g = np.linalg.inv(1j * w[iw] * np.eye(nb) - H[ik])
-            out[:, :, 0, ik, iw] = g
-            out[:, :, 1, ik, iw] = g
+    for s in range(2):
+        for ik in range(nc):
+            for iw in range(n_ext):
+                out[:, :, s, ik, iw] = np.linalg.inv((1j * w[iw] - h[s]) * np.eye(nb) - H[ik])
     return out
 
 
@@ -238,10 +241,10 @@ class Geometry:
 
 def make_geometry(nb: int, cluster: Cluster, beta: float, n_freq: int, w_ex: Sequence[int],
                   k_ex: Sequence[int], channels: Sequence[int],
-                  g0: np.ndarray | None = None) -> Geometry:
+                  g0: np.ndarray | None = None, zeeman: float = 0.0) -> Geometry:
     w_ex = [int(w) for w in w_ex]
     ext = extension_size(w_ex)
     if g0 is None:
-        g0 = free_g0(cluster, nb, beta, n_freq + 2 * ext)
+        g0 = free_g0(cluster, nb, beta, n_freq + 2 * ext, zeeman)
     return Geometry(nb=nb, cluster=cluster, beta=float(beta), n_freq=int(n_freq), w_ex=w_ex,
                     k_ex=[int(k) for k in k_ex], channels=sort_channels(channels), g0=g0)

@@ -117,6 +117,17 @@ class Workload:
         return [draw_measurement(rs, g.nb, g.nc, g.beta, self.draw_n, self.signed)
                 for _ in range(count)]
 
+    def draw_chunks(self, count: int, chunk: int, seed_offset: int = 0):
+        """The same stream as draw(count), yielded in lists of at most `chunk` measurements (so a
+        long run never holds all of its M matrices at once)."""
+        rs = self.stream(seed_offset)
+        g = self.geometry
+        done = 0
+        while done < count:
+            k = min(chunk, count - done)
+            yield [draw_measurement(rs, g.nb, g.nc, g.beta, self.draw_n, self.signed) for _ in range(k)]
+            done += k
+
 
 def _poisson_clipped(lam, lo, hi):
     return lambda rs: min(max(rs.poisson(lam), lo), hi)
@@ -173,13 +184,13 @@ def workload(cfg: int) -> Workload:
 def tiny_workload(nb: int = 1, L: int = 2, n_freq: int = 4, beta: float = 2.0,
                   w_ex=(0,), k_ex=((0.0, 0.0),), channels=None, n_range=(1, 6),
                   signed: bool = True, cluster: geo.Cluster | None = None,
-                  seed_cfg: int = 100) -> Workload:
+                  seed_cfg: int = 100, zeeman: float = 0.0) -> Workload:
     """Small workloads for parity tests (sizes the oracle finishes in well under a second)."""
     cl = cluster if cluster is not None else geo.square_cluster(L)
     if channels is None:
         channels = [geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE,
                     geo.PARTICLE_PARTICLE_UP_DOWN]
     kex = [cl.k_index(k) for k in k_ex]
-    g = geo.make_geometry(nb, cl, beta, n_freq, list(w_ex), kex, channels)
+    g = geo.make_geometry(nb, cl, beta, n_freq, list(w_ex), kex, channels, zeeman=zeeman)
     lo, hi = n_range
     return Workload(seed_cfg, g, _uniform_n(lo, hi), signed, 1, hi, 4, "tiny")

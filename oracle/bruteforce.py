@@ -9,7 +9,9 @@ they evaluate the defining sums, so they pin the oracle's NDFT/DFT/G steps indep
       f(b1, b2, r1, r2, w1, w2) = sum_{i in (b1,r1), j in (b2,r2)} M_ij exp(i (t_i w1 - t_j w2))
 * Cluster DFT (space_transform_2D.hpp:41-44): f(k1, k2) = sum_{r1,r2} exp(i(k1 r1 - k2 r2)) f(r1,r2) / Nc
 * G = -G0 M G0 + beta G0 delta (tp_accumulator_cpu.hpp:308-375) as a plain band-matrix product.
-* Closed forms of G4 for M = 0 (G = beta G0 delta) and a spin-independent G0.
+* Closed forms of G4 for M = 0 (G = beta G0 delta), with a spin-independent or Zeeman-split G0.
+* G of a single vertex per spin (diagonal plus a non-symmetric rank-one part) in closed form, and
+  the G4 channel definitions evaluated on a given G (g4_definition).
 """
 from __future__ import annotations
 
@@ -82,46 +84,172 @@ def logical_G(geom: geo.Geometry, G: np.ndarray) -> np.ndarray:
 def g4_free(geom: geo.Geometry, channel: int, sign: float) -> np.ndarray:
     """G4 of one measurement with M = 0 on both spins, from closed forms, shaped
     (b1, b2, b3, b4, k1, w1, k2, w2, k_ex, w_ex) (Fortran order). With M = 0,
-    G_s(b,b';k1,k2;w1,w2) = beta G0(b',b;k1,w1) delta_{k1k2} delta_{w1w2} (getGMultiband's
-    band-transposed read), G0 spin independent, so (a = sign/2, sum over both spins):
-      PP      :  sign beta^2 G0(b3,b1;k,w) G0(b4,b2;q-k,v-w)                    on k1=k2, w1=w2
-      MAG     : -sign beta^2 G0(b1,b3;k,w) G0(b4,b2;k+q,w+v)                    on k1=k2, w1=w2
-      CHARGE  : -sign beta^2 G0(b4,b1;k,w) G0(b3,b2;k+q,w+v)                    on k1=k2, w1=w2
-                + 2 sign beta^2 G0(b1,b2;k1,w1) G0(b4,b3;k2,w2)                 if q = 0, v = 0
-      TRANSV. : -sign beta^2 G0(b4,b2;k,w) G0(b1,b3;k+q,w+v)                    on k1=k2, w1=w2
-    (TRANSVERSE from -a sum_s G_s(b2,b4;k1,k2) G_{-s}(b3,b1;k2+q,k1+q), tp_accumulator_cpu.hpp:454-489.)
+    G_s(b,b';k1,k2;w1,w2) = beta g_s(b',b;k1,w1) delta_{k1k2} delta_{w1w2} (getGMultiband's
+    band-transposed read of G_ = beta G0 delta), g_s = G0(., ., s, k, w), so with a = sign/2,
+    sigma_0 = +1, sigma_1 = -1 and p + Q = (k + q, w + v), Q - p = (q - k, v - w):
+      PP      :  a beta^2 sum_s g_s(b3,b1;p) g_-s(b4,b2;Q-p)                        on p1 = p2
+      MAG     : -a beta^2 sum_s g_s(b1,b3;p) g_s(b4,b2;p+Q)                         on p1 = p2
+                + a beta^2 [sum_s sigma_s g_s(b1,b2;p1)] [sum_s sigma_s g_s(b4,b3;p2)]  if Q = 0
+      CHARGE  : -a beta^2 sum_s g_s(b4,b1;p) g_s(b3,b2;p+Q)                         on p1 = p2
+                + a beta^2 [sum_s g_s(b1,b2;p1)] [sum_s g_s(b4,b3;p2)]                if Q = 0
+      TRANSV. : -a beta^2 sum_s g_s(b4,b2;p) g_-s(b1,b3;p+Q)                        on p1 = p2
+    from the channel definitions of tp_accumulator_cpu.hpp:454-661 (the pp, transverse and
+    crossed terms pair one factor per spin; the magnetic direct term weighs the spins by
+    sigma_s1 sigma_s2). A spin-independent G0 makes the magnetic direct term vanish and the s / -s
+    pairings coincide; a Zeeman-split G0 (geometry.free_g0) keeps them apart.
     """
     nb, nc, nw = geom.nb, geom.nc, geom.n_freq
     ext = geom.ext
     add, sub = geom.cluster.add, geom.cluster.sub
-    g0 = geom.g0[:, :, 0]  # spin independent (b, b', k, wE)
     b2 = geom.beta ** 2
+    a = 0.5 * sign
     out = np.zeros(geom.g4_shape(), complex)
     gamma = geom.cluster.k_index((0.0, 0.0))
     ks = np.arange(nc)
     ws = np.arange(nw)
-    g1 = g0[:, :, :, ext:ext + nw]  # (b, b', k, w) at the compact frequencies
+    g = [geom.g0[:, :, s] for s in range(2)]            # (b, b', k, wE) per spin
+    g1 = [x[:, :, :, ext:ext + nw] for x in g]          # (b, b', k, w) at the compact frequencies
+    sigma = (1.0, -1.0)
     for iv, nu in enumerate(geom.w_ex):
         for iq, q in enumerate(geom.k_ex):
-            if channel == geo.PARTICLE_PARTICLE_UP_DOWN:
-                g2 = g0[:, :, sub[ks, q]][:, :, :, nu + nw - 1 - ws + ext]   # (b, b', k, w)
-                val = sign * b2 * np.einsum("cakw,dbkw->abcdkw", g1, g2)
-            elif channel == geo.PARTICLE_HOLE_MAGNETIC:
-                g2 = g0[:, :, add[ks, q]][:, :, :, ws + nu + ext]
-                val = -sign * b2 * np.einsum("ackw,dbkw->abcdkw", g1, g2)
-            elif channel == geo.PARTICLE_HOLE_CHARGE:
-                g2 = g0[:, :, add[ks, q]][:, :, :, ws + nu + ext]
-                val = -sign * b2 * np.einsum("dakw,cbkw->abcdkw", g1, g2)
-            elif channel == geo.PARTICLE_HOLE_TRANSVERSE:
-                g2 = g0[:, :, add[ks, q]][:, :, :, ws + nu + ext]
-                val = -sign * b2 * np.einsum("dbkw,ackw->abcdkw", g1, g2)
-            else:
-                raise ValueError("no closed form for this channel")
+            val = np.zeros((nb, nb, nb, nb, nc, nw), complex)
+            for s in range(2):
+                t = 1 - s
+                if channel == geo.PARTICLE_PARTICLE_UP_DOWN:
+                    g2 = g[t][:, :, sub[ks, q]][:, :, :, nu + nw - 1 - ws + ext]
+                    val += a * b2 * np.einsum("cakw,dbkw->abcdkw", g1[s], g2)
+                elif channel == geo.PARTICLE_HOLE_MAGNETIC:
+                    g2 = g[s][:, :, add[ks, q]][:, :, :, ws + nu + ext]
+                    val -= a * b2 * np.einsum("ackw,dbkw->abcdkw", g1[s], g2)
+                elif channel == geo.PARTICLE_HOLE_CHARGE:
+                    g2 = g[s][:, :, add[ks, q]][:, :, :, ws + nu + ext]
+                    val -= a * b2 * np.einsum("dakw,cbkw->abcdkw", g1[s], g2)
+                elif channel == geo.PARTICLE_HOLE_TRANSVERSE:
+                    g2 = g[t][:, :, add[ks, q]][:, :, :, ws + nu + ext]
+                    val -= a * b2 * np.einsum("dbkw,ackw->abcdkw", g1[s], g2)
+                else:
+                    raise ValueError("no closed form for this channel")
             for k in range(nc):
                 out[:, :, :, :, k, :, k, :, iq, iv][..., ws, ws] += val[..., k, :]
-            if channel == geo.PARTICLE_HOLE_CHARGE and q == gamma and nu == 0:
-                out[..., iq, iv] += 2 * sign * b2 * np.einsum("abkw,dclv->abcdkwlv", g1, g1)
+            if q == gamma and nu == 0 and channel in (geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE):
+                wgt = sigma if channel == geo.PARTICLE_HOLE_MAGNETIC else (1.0, 1.0)
+                d = wgt[0] * g1[0] + wgt[1] * g1[1]
+                out[..., iq, iv] += a * b2 * np.einsum("abkw,dclv->abcdkwlv", d, d)
     return out
+
+
+def g_single_vertex(geom: geo.Geometry, vertices) -> np.ndarray:
+    """G in the reference layout (b1, b2, s, k1, k2, w1E, w2E) for an M with at most one vertex
+    per spin sector, from the closed form of its NDFT: a vertex (tau, r, b) with M = (m) gives
+    M~_s(b1,b2;k1,k2;w1,w2) = (m / Nc) [b1 = b][b2 = b] exp(i (k1 - k2).r + i (w1 - w2) tau), so
+      G_s(b1, b2) = beta G0_s(b1,b2;p1) delta_{p1p2}
+                    - (m / Nc) G0_s(b1,b;p1) exp(i(k1.r + w1 tau)) G0_s(b,b2;p2) exp(-i(k2.r + w2 tau)),
+    a diagonal plus a rank-one matrix that is not symmetric, so every off-diagonal G read of the
+    G4 update sees its argument order. vertices[s] = (tau, site, band, m) or None."""
+    nb, nc, nE = geom.nb, geom.nc, geom.n_ext
+    w = geo.vertex_frequencies(nE, geom.beta)
+    G = np.zeros((nb, nb, 2, nc, nc, nE, nE), complex)
+    for s in range(2):
+        g0 = geom.g0[:, :, s]  # (b, b', k, w)
+        for k in range(nc):
+            for i in range(nE):
+                G[:, :, s, k, k, i, i] += geom.beta * g0[:, :, k, i]
+        if vertices[s] is None:
+            continue
+        tau, site, band, m = vertices[s]
+        ph = np.exp(1j * (geom.cluster.k @ geom.cluster.r[site])[:, None] + 1j * w[None, :] * tau)  # (k, w)
+        left = g0[:, band] * ph[None]              # (b1, k1, w1)
+        right = g0[band, :] * np.conj(ph)[None]    # (b2, k2, w2)
+        G[:, :, s] -= (m / nc) * np.einsum("akw,blv->abklwv", left, right)
+    return G
+
+
+def g4_definition(geom: geo.Geometry, G: np.ndarray, channel: int, sign: float,
+                  variant: str = "") -> np.ndarray:
+    """G4 of one measurement from its G (reference layout), by the channel definitions, shaped
+    like g4_free. With L_s(b, b'; p1, p2) = G(b', b, s, k1, k2, w1 + ext, w2 + ext) (getGMultiband's
+    band-transposed read, tp_accumulator_cpu.hpp:388-405), a = sign / 2, p + Q = (k + q, w + v) and
+    Q - p = (q - k, v + nw - 1 - w):
+      MAG    : a sum_{s1,s2} sigma_s1 sigma_s2 L_s1(b2,b1; p1, p1+Q) L_s2(b3,b4; p2+Q, p2)
+               - a sum_s L_s(b3,b1; p1, p2) L_s(b2,b4; p2+Q, p1+Q)
+      CHARGE : a sum_{s1,s2} L_s1(b2,b1; p1, p1+Q) L_s2(b3,b4; p2+Q, p2)
+               - a sum_s L_s(b1,b4; p1, p2) L_s(b2,b3; p2+Q, p1+Q)
+      TRANSV.: -a sum_s L_s(b2,b4; p1, p2) L_-s(b3,b1; p2+Q, p1+Q)
+      PP     : a sum_s L_s(b1,b3; p1, p2) L_-s(b2,b4; Q-p1, Q-p2)
+    For one band the momentum and frequency arguments of the ph channels are the Wick
+    contractions written in the reference's comments (tp_accumulator_cpu.hpp:455-456, 491-495,
+    549-553): <c+(k1+q) c(k1) c+(k2) c(k2+q)> = G(k1, k1+q) G(k2+q, k2) - G(k2+q, k1+q) G(k1, k2)
+    with G(x, y) ~ <c(x) c+(y)>. For pp the comment (:629-630) reads G(q-k2, q-k1, s) G(k2, k1, -s),
+    the transpose of both factors; the code (:640-648), and so every result of the reference,
+    contracts G(k1, k2, s) G(q-k1, q-k2, -s), which is what is written here. The band slots of the
+    several-band terms are those of the code (:513-523, 532-543, 561-572, 640-648).
+
+    variant (test-only planted mistakes, to show that the checks see them): "pp_comment" (the pp
+    comment's argument order), "crossed_transposed" (G(p1+Q, p2+Q) in the crossed ph term),
+    "transverse_same_spin" (s instead of -s), "mag_spin_sum" (no sigma weights in the magnetic
+    direct term)."""
+    nb, nc, nw = geom.nb, geom.nc, geom.n_freq
+    ext = geom.ext
+    add, sub = geom.cluster.add, geom.cluster.sub
+    L = np.transpose(G, (2, 1, 0, 3, 4, 5, 6))  # (s, b, b', k1, k2, w1E, w2E)
+    a = 0.5 * sign
+    sigma = (1.0, -1.0) if variant != "mag_spin_sum" else (1.0, 1.0)
+    K1 = np.arange(nc)[:, None, None, None]
+    W1 = np.arange(nw)[None, :, None, None]
+    K2 = np.arange(nc)[None, None, :, None]
+    W2 = np.arange(nw)[None, None, None, :]
+
+    def read(s, kA, kB, wA, wB):
+        kA, kB, wA, wB = np.broadcast_arrays(kA, kB, wA, wB)
+        return L[s][:, :, kA, kB, wA + ext, wB + ext]  # (b, b', k1, w1, k2, w2)
+
+    out = np.zeros(geom.g4_shape(), complex)
+    for iv, nu in enumerate(geom.w_ex):
+        for iq, q in enumerate(geom.k_ex):
+            K1q, K2q = add[K1, q], add[K2, q]
+            val = np.zeros((nb, nb, nb, nb, nc, nw, nc, nw), complex)
+            if channel in (geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE):
+                wgt = sigma if channel == geo.PARTICLE_HOLE_MAGNETIC else (1.0, 1.0)
+                A = sum(wgt[s] * read(s, K1, K1q, W1, W1 + nu) for s in range(2))
+                B = sum(wgt[s] * read(s, K2q, K2, W2 + nu, W2) for s in range(2))
+                val += a * np.einsum("ba...,cd...->abcd...", A, B)
+                for s in range(2):
+                    X = read(s, K1, K2, W1, W2)
+                    if variant == "crossed_transposed":
+                        Y = read(s, K1q, K2q, W1 + nu, W2 + nu)
+                    else:
+                        Y = read(s, K2q, K1q, W2 + nu, W1 + nu)
+                    if channel == geo.PARTICLE_HOLE_MAGNETIC:
+                        val -= a * np.einsum("ca...,bd...->abcd...", X, Y)
+                    else:
+                        val -= a * np.einsum("ad...,bc...->abcd...", X, Y)
+            elif channel == geo.PARTICLE_HOLE_TRANSVERSE:
+                for s in range(2):
+                    t = s if variant == "transverse_same_spin" else 1 - s
+                    X = read(s, K1, K2, W1, W2)
+                    Y = read(t, K2q, K1q, W2 + nu, W1 + nu)
+                    val -= a * np.einsum("bd...,ca...->abcd...", X, Y)
+            elif channel == geo.PARTICLE_PARTICLE_UP_DOWN:
+                mK1, mK2 = sub[K1, q], sub[K2, q]
+                mW1, mW2 = nu + nw - 1 - W1, nu + nw - 1 - W2
+                for s in range(2):
+                    if variant == "pp_comment":
+                        X = read(s, mK2, mK1, mW2, mW1)
+                        Y = read(1 - s, K2, K1, W2, W1)
+                    else:
+                        X = read(s, K1, K2, W1, W2)
+                        Y = read(1 - s, mK1, mK2, mW1, mW2)
+                    val += a * np.einsum("ac...,bd...->abcd...", X, Y)
+            else:
+                raise ValueError("unknown channel")
+            out[..., iq, iv] = val
+    return out
+
+
+def normalize_g4(g4: np.ndarray, beta: float, accumulated_sign: float) -> np.ndarray:
+    """The solvers' G4 normalisation after the accumulation: G4 / (accumulated_sign beta^2)
+    (ctaux_cluster_solver.hpp:523-527, ctint_cluster_solver.hpp:511-516), in FP64."""
+    return np.asarray(g4, complex) / (accumulated_sign * beta * beta)
 
 
 def difference(ref: np.ndarray, x: np.ndarray) -> dict:

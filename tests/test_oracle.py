@@ -7,7 +7,13 @@ sign, index or conjugation, a transposed band pair) fails it:
     of 100 eps (test/unit/.../tp/ndft/cached_ndft_gpu_test.cpp:68);
   * the rank-1 M of the reference's cached_ndft_test.hpp:100-118, whose NDFT factorises;
   * a single vertex (closed form) and the full direct M -> G definition;
-  * M = 0 closed forms of every channel, for one and three bands, with non-zero transfers;
+  * M = 0 closed forms of every channel, for one and three bands, with non-zero transfers, with a
+    spin-independent and a Zeeman-split G0 (the latter keeps the magnetic spin-difference term and
+    the s / -s pairings of the transverse and pp channels from vanishing or coinciding);
+  * one vertex per spin: G in closed form (diagonal plus a non-symmetric rank-one part) and G4 by
+    the channel definitions, which sees the argument order of every off-diagonal G read;
+  * planted mistakes in those closed forms (argument order, spin pairing, spin weights, the pp
+    comment's transposed order) are far from the oracle, so the checks can fail;
   * linearity in the sign, additivity, sumTo, the empty configuration;
   * the conjugation symmetry of a real M's NDFT.
 """
@@ -115,13 +121,16 @@ def test_real_M_conjugation_symmetry(oracle_lib):
     assert bf.difference(mr, np.conj(flipped))["l_inf"] < 100 * EPS
 
 
+ALL_CHANNELS = [geo.PARTICLE_HOLE_TRANSVERSE, geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE,
+                geo.PARTICLE_PARTICLE_UP_DOWN]
+
+
 @pytest.mark.parametrize("nb,L", [(1, 2), (1, 3), (3, 2)])
 @pytest.mark.parametrize("sign", [1.0, -1.0])
-def test_free_closed_forms(oracle_lib, nb, L, sign):
+@pytest.mark.parametrize("zeeman", [0.0, 0.37])
+def test_free_closed_forms(oracle_lib, nb, L, sign, zeeman):
     k_ex = ((0.0, 0.0), (math.pi, math.pi)) if L == 2 else ((0.0, 0.0), (2 * math.pi / 3, 0.0))
-    wl = _wl(nb=nb, L=L, n_freq=4, w_ex=(0, 1, 2), k_ex=k_ex,
-             channels=[geo.PARTICLE_HOLE_TRANSVERSE, geo.PARTICLE_HOLE_MAGNETIC,
-                       geo.PARTICLE_HOLE_CHARGE, geo.PARTICLE_PARTICLE_UP_DOWN])
+    wl = _wl(nb=nb, L=L, n_freq=4, w_ex=(0, 1, 2), k_ex=k_ex, channels=ALL_CHANNELS, zeeman=zeeman)
     g = wl.geometry
     meas = wl.draw(1)[0]
     meas.M = [np.zeros_like(x) for x in meas.M]
@@ -191,3 +200,126 @@ def test_empty_configuration_is_noop(oracle_lib):
                               sign=1.0)
     assert a.accumulate(empty) == 0.0
     assert all(np.all(x == 0) for x in a.get_G4())
+
+
+def _one_vertex_each(g, rs):
+    """One vertex per spin at distinct (tau, site, band) with distinct M values, of a size (~Nc)
+    that makes the rank-one part of G comparable with beta G0."""
+    verts, n, Ms, taus, sites, bands = [], [], [], [], [], []
+    for s in range(2):
+        u = rs.uniform(4)
+        tau, site, band, m = u[0] * g.beta, int(u[1] * g.nc), int(u[2] * g.nb), g.nc * (1.0 + u[3])
+        verts.append((tau, site, band, m))
+        n.append(1)
+        Ms.append(np.full((1, 1), m, order="F"))
+        taus.append(np.array([tau]))
+        sites.append(np.array([site], np.int32))
+        bands.append(np.array([band], np.int32))
+    return verts, synth.Measurement(n=n, M=Ms, tau=taus, site=sites, band=bands, sign=-1.0)
+
+
+SINGLE_VERTEX_CASES = {
+    # name: (nb, L, w_ex, k_ex)
+    "1band_2x2": (1, 2, (0, 1, 2), ((0.0, 0.0), (math.pi, 0.0))),
+    "1band_3x3": (1, 3, (0, 2), ((2 * math.pi / 3, 0.0), (0.0, 2 * math.pi / 3))),
+    "1band_3x3_negative_wex": (1, 3, (-1,), ((2 * math.pi / 3, 2 * math.pi / 3),)),
+    "3band_2x2": (3, 2, (0, 1), ((0.0, 0.0), (math.pi, math.pi))),
+}
+
+
+def _single_vertex_case(oracle_lib, name, seed):
+    nb, L, w_ex, k_ex = SINGLE_VERTEX_CASES[name]
+    wl = _wl(nb=nb, L=L, n_freq=4, w_ex=w_ex, k_ex=k_ex, channels=ALL_CHANNELS, zeeman=0.41)
+    g = wl.geometry
+    verts, meas = _one_vertex_each(g, synth.Stream(seed))
+    acc = oracle_lib.OracleAccumulator(g)
+    acc.accumulate(meas)
+    return g, verts, meas, acc
+
+
+@pytest.mark.parametrize("name", list(SINGLE_VERTEX_CASES))
+def test_single_vertex_G_closed_form(oracle_lib, name):
+    g, verts, meas, acc = _single_vertex_case(oracle_lib, name, 7)
+    ref = bf.g_single_vertex(g, verts)
+    assert bf.difference(ref, acc.get_G())["l_inf"] < 1e-13
+    # the rank-one part is not symmetric, so a transposed read of it would differ
+    r1 = (ref - bf.g_single_vertex(g, [None, None]))[:, :, 0]  # (b1, b2, k1, k2, w1, w2)
+    assert np.abs(r1).max() > 0
+    r1t = np.transpose(r1, (1, 0, 3, 2, 5, 4))
+    assert np.abs(r1 - r1t).max() > 0.1 * np.abs(r1).max()
+
+
+@pytest.mark.parametrize("name", list(SINGLE_VERTEX_CASES))
+@pytest.mark.parametrize("seed", [11, 12])
+def test_single_vertex_g4_definition(oracle_lib, name, seed):
+    """The oracle's accumulate of one vertex per spin (Zeeman-split G0, non-zero transfers)
+    against the channel definitions evaluated on the closed-form G."""
+    g, verts, meas, acc = _single_vertex_case(oracle_lib, name, seed)
+    G = bf.g_single_vertex(g, verts)
+    for c, ch in enumerate(g.channels):
+        ref = bf.g4_definition(g, G, ch, meas.sign).ravel(order="F")
+        err = bf.difference(ref, acc.get_G4()[c])["l_inf"]
+        assert err < 1e-13, (geo.CHANNEL_NAMES[ch], err)
+
+
+def test_free_closed_form_equals_definition():
+    """The M = 0 closed forms and the channel definitions on G = beta G0 delta agree (two
+    independent statements of the same sums)."""
+    for nb in (1, 3):
+        g = _wl(nb=nb, L=2, n_freq=4, w_ex=(0, 1), channels=ALL_CHANNELS, zeeman=0.29).geometry
+        G = bf.g_single_vertex(g, [None, None])
+        for ch in g.channels:
+            a = bf.g4_free(g, ch, 1.0)
+            b = bf.g4_definition(g, G, ch, 1.0)
+            assert bf.difference(a, b)["l_inf"] < 1e-14
+
+
+@pytest.mark.parametrize("variant,channel", [
+    ("pp_comment", geo.PARTICLE_PARTICLE_UP_DOWN),
+    ("crossed_transposed", geo.PARTICLE_HOLE_MAGNETIC),
+    ("crossed_transposed", geo.PARTICLE_HOLE_CHARGE),
+    ("transverse_same_spin", geo.PARTICLE_HOLE_TRANSVERSE),
+    ("mag_spin_sum", geo.PARTICLE_HOLE_MAGNETIC),
+])
+@pytest.mark.parametrize("name", ["1band_3x3", "3band_2x2"])
+def test_definition_checks_detect_planted_mistakes(oracle_lib, name, variant, channel):
+    """Each plausible mistake, planted in the closed-form side on the CPU, is far from the oracle:
+    the single-vertex check pins argument order, spin pairing and spin weights."""
+    g, verts, meas, acc = _single_vertex_case(oracle_lib, name, 11)
+    G = bf.g_single_vertex(g, verts)
+    c = g.channels.index(channel)
+    got = acc.get_G4()[c]
+    right = bf.g4_definition(g, G, channel, meas.sign).ravel(order="F")
+    wrong = bf.g4_definition(g, G, channel, meas.sign, variant=variant).ravel(order="F")
+    assert bf.difference(right, got)["l_inf"] < 1e-13
+    assert bf.difference(wrong, got)["l_inf"] > 1e-3
+
+
+def test_zeeman_free_forms_see_spin_structure(oracle_lib):
+    """With a Zeeman-split G0 and M = 0, the magnetic direct term is non-zero and the s / -s
+    pairings differ, so a spin-independent closed form no longer matches the oracle."""
+    wl = _wl(nb=3, L=2, n_freq=4, w_ex=(0, 1), channels=ALL_CHANNELS, zeeman=0.37)
+    g = wl.geometry
+    meas = wl.draw(1)[0]
+    meas.M = [np.zeros_like(x) for x in meas.M]
+    acc = oracle_lib.OracleAccumulator(g)
+    acc.accumulate(meas)
+    g_sym = _wl(nb=3, L=2, n_freq=4, w_ex=(0, 1), channels=ALL_CHANNELS).geometry
+    g_sym.g0 = np.repeat(g.g0[:, :, :1], 2, axis=2)  # spin slot 0 for both spins
+    for c, ch in enumerate(g.channels):
+        exact = bf.g4_free(g, ch, meas.sign).ravel(order="F")
+        assert bf.difference(exact, acc.get_G4()[c])["l_inf"] < 1e-14
+        if ch != geo.PARTICLE_HOLE_CHARGE:
+            sym = bf.g4_free(g_sym, ch, meas.sign).ravel(order="F")
+            assert bf.difference(sym, acc.get_G4()[c])["l_inf"] > 1e-3, geo.CHANNEL_NAMES[ch]
+
+
+def test_normalize_g4():
+    """G4 / (accumulated sign beta^2) (ctaux_cluster_solver.hpp:523-527), with mixed signs: two
+    measurements of sign +1 and one of -1 leave an accumulated sign of 1."""
+    rs = synth.Stream(99)
+    g4 = rs.normal(64) + 1j * rs.normal(64)
+    out = bf.normalize_g4(g4, 4.0, 1.0)
+    assert np.allclose(out * 16.0, g4, rtol=1e-15, atol=0)
+    out = bf.normalize_g4(g4, 2.0, -3.0)
+    assert np.allclose(out * -12.0, g4, rtol=1e-15, atol=0)
