@@ -33,14 +33,24 @@ CASES = {
     "1band_8site": (1, "8site", 6, (0, 1), ((0.0, 0.0), (math.pi, math.pi)), (4, 20)),
     "1band_6x6": (1, 6, 2, (0, 1), ((0.0, 0.0), (math.pi, 0.0), (math.pi, math.pi),
                                     (2 * math.pi / 3, 0.0)), (10, 60)),
+    # the geometries of the 8x8 and three-band workloads, at reduced frequency counts: the Nc = 64
+    # M -> G instantiations and the 1024-thread, four-pairs-per-thread G4 tile; the three-band
+    # 8-site tile and its FP32-Kahan sum
+    "1band_8x8": (1, 8, 2, (0,), ((0.0, 0.0), (math.pi, math.pi)), (10, 60)),
+    "3band_8site": (3, "8site", 4, (0, 1), ((0.0, 0.0), (math.pi, math.pi)), (4, 30)),
+    # a Zeeman-split G0: the spin sectors differ, so the magnetic spin-difference term and the
+    # s / -s pairings of the transverse and pp channels are exercised
+    "1band_4x4_zeeman": (1, 4, 4, (0, 1), ((0.0, 0.0), (math.pi, 0.0)), (8, 30), 0.37),
+    "3band_2x2_zeeman": (3, 2, 4, (0, 1), ((0.0, 0.0), (math.pi, math.pi)), (2, 10), 0.37),
 }
 
 
 def case(name, channels=ALL_CHANNELS):
-    nb, L, nf, wex, kex, nr = CASES[name]
+    nb, L, nf, wex, kex, nr = CASES[name][:6]
+    zeeman = CASES[name][6] if len(CASES[name]) > 6 else 0.0
     cl = geo.cluster_2d((2, 2), (2, -2)) if L == "8site" else geo.square_cluster(L)
     return synth.tiny_workload(nb=nb, cluster=cl, n_freq=nf, w_ex=wex, k_ex=kex, channels=channels,
-                               n_range=nr, seed_cfg=200 + list(CASES).index(name))
+                               n_range=nr, seed_cfg=200 + list(CASES).index(name), zeeman=zeeman)
 
 
 def linf(ref, x):
@@ -102,6 +112,57 @@ def test_accumulate_matches_oracle(oracle_lib, name, precision, sum_mode, tol):
     for ch, r, x in zip(g.channels, ref, acc.get_G4()):
         err = linf(r, x)
         print(f"{name} {precision}/{sum_mode} {geo.CHANNEL_NAMES[ch]} l_inf={err:.3e}", flush=True)
+        assert err < tol
+
+
+@pytest.mark.parametrize("name", ["1band_8x8", "3band_8site", "1band_4x4_zeeman", "3band_2x2_zeeman",
+                                  "1band_6x6", "1band_4x4_4q"])
+@pytest.mark.parametrize("precision,sum_mode,tol", [("fp64", "fp64", 1e-12), ("fp32", "fp64", 1e-6),
+                                                     ("fp32", "fp32_kahan", 1e-6)])
+def test_batched_matches_oracle(oracle_lib, name, precision, sum_mode, tol):
+    """A batch of 9 walkers in launches of 4 (the batched tile kernels, their XCD-aware schedule,
+    the rank-one GEMM where it is on by default) against the oracle's sequential sum."""
+    wl = case(name)
+    g = wl.geometry
+    ms = wl.draw(9)
+    ref = oracle_g4(oracle_lib, g, ms)
+    acc = g4acc.TpAccumulator(g, precision=precision, sum_mode=sum_mode, max_walkers=4,
+                              max_vertices=wl.n_max)
+    acc.resetAccumulation()
+    acc.accumulate_batch(ms)
+    for ch, r, x in zip(g.channels, ref, acc.get_G4()):
+        err = linf(r, x)
+        print(f"{name} batched {precision}/{sum_mode} {geo.CHANNEL_NAMES[ch]} l_inf={err:.3e}", flush=True)
+        assert err < tol
+
+
+@pytest.mark.parametrize("name", ["1band_4x4_zeeman", "3band_2x2_zeeman", "1band_8x8"])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-6)])
+def test_single_vertex_matches_definition(name, precision, tol):
+    """One vertex per spin: G4 of the HIP path against the channel definitions evaluated on the
+    closed-form G (oracle/bruteforce.py), independently of the oracle's loops."""
+    wl = case(name)
+    g = wl.geometry
+    rs = synth.Stream(4242)
+    verts, n, Ms, taus, sites, bands = [], [], [], [], [], []
+    for s in range(2):
+        u = rs.uniform(4)
+        v = (u[0] * g.beta, int(u[1] * g.nc), int(u[2] * g.nb), g.nc * (1.0 + u[3]))
+        verts.append(v)
+        n.append(1)
+        Ms.append(np.full((1, 1), v[3], order="F"))
+        taus.append(np.array([v[0]]))
+        sites.append(np.array([v[1]], np.int32))
+        bands.append(np.array([v[2]], np.int32))
+    meas = synth.Measurement(n=n, M=Ms, tau=taus, site=sites, band=bands, sign=-1.0)
+    G = bf.g_single_vertex(g, verts)
+    acc = g4acc.TpAccumulator(g, precision=precision, max_vertices=4)
+    acc.resetAccumulation()
+    acc.accumulate(meas)
+    for ch, x in zip(g.channels, acc.get_G4()):
+        ref = bf.g4_definition(g, G, ch, meas.sign).ravel(order="F")
+        err = linf(ref, x)
+        print(f"{name} single vertex {precision} {geo.CHANNEL_NAMES[ch]} l_inf={err:.3e}", flush=True)
         assert err < tol
 
 
