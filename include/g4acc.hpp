@@ -9,10 +9,14 @@
 // Template arguments are duck-typed on the reference's types:
 //   Matrix         nrRows(), leadingDimension(), ptr()           (dca::linalg::Matrix<Real, ...>)
 //   Configuration  size(), operator[](i).get_tau(),
-//                  operator[](i).get_right_band(), .get_right_site()   (CT-AUX vertex_singleton,
+//                  operator[](i).get_right_band(), .get_right_site(),
+//                  operator[](i).get_left_band(), .get_left_site()    (CT-AUX vertex_singleton,
 //                  ctaux/structs/vertex_singleton.hpp:84-95; CT-INT sector entries)
 // The orbital of a vertex is taken from its right band and site, as CachedNdftBase does for
-// density-density interactions (tp/ndft/cached_ndft_base.hpp:97-103).
+// density-density interactions (tp/ndft/cached_ndft_base.hpp:97-103). A vertex whose left and
+// right orbitals differ (a non-density-density interaction, which the reference handles with a
+// second sorted side, cached_ndft_base.hpp:72-78) raises std::logic_error instead of giving a
+// wrong G4.
 #ifndef G4ACC_HPP
 #define G4ACC_HPP
 
@@ -41,6 +45,51 @@ inline void throw_on_error(int status, const char* message) {
     default:
       throw std::logic_error(message ? message : "g4acc error");
   }
+}
+
+// Vertex times and orbitals of one spin sector's configuration, as the accumulator reads them
+// (CachedNdftBase::sortConfiguration, tp/ndft/cached_ndft_base.hpp:89-129, density-density:
+// both sides use the right band and site). Throws std::logic_error for a vertex whose left and
+// right band or site differ.
+template <class Configuration>
+void extract_configuration(const Configuration& conf, std::vector<double>& tau, std::vector<int>& site,
+                           std::vector<int>& band) {
+  const int n = static_cast<int>(conf.size());
+  tau.resize(n);
+  site.resize(n);
+  band.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const auto& v = conf[i];
+    if (v.get_left_band() != v.get_right_band() || v.get_left_site() != v.get_right_site())
+      throw std::logic_error(
+          "g4acc: vertex " + std::to_string(i) +
+          " has different left and right orbitals; only density-density vertices are supported");
+    tau[i] = v.get_tau();
+    band[i] = v.get_right_band();
+    site[i] = v.get_right_site();
+  }
+}
+
+// G0 over the EXTENDED vertex frequencies, as TpAccumulatorBase::initializeG0 slices the
+// cluster-excluded single-particle G0 (tp_accumulator_base.hpp:210-221), in the (b1, b2, s, k, wE)
+// layout of g4acc_geometry::g0. g0_sp(b1, s1, b2, s2, k, w) reads the single-particle function
+// over its n_sp_freq frequencies. Throws std::logic_error when there are fewer single-particle
+// than extended vertex frequencies, as the reference constructor does (:197-198).
+template <class G0Function>
+std::vector<std::complex<double>> slice_g0(const G0Function& g0_sp, int n_bands, int n_sites,
+                                           int n_sp_freq, int n_ext_freq) {
+  if (n_sp_freq < n_ext_freq)
+    throw std::logic_error("The number of single particle frequencies is too small.");
+  const int off = (n_sp_freq - n_ext_freq) / 2;
+  std::vector<std::complex<double>> g0;
+  g0.reserve(static_cast<std::size_t>(n_bands) * n_bands * 2 * n_sites * n_ext_freq);
+  for (int w = 0; w < n_ext_freq; ++w)
+    for (int k = 0; k < n_sites; ++k)
+      for (int s = 0; s < 2; ++s)
+        for (int b2 = 0; b2 < n_bands; ++b2)
+          for (int b1 = 0; b1 < n_bands; ++b1)
+            g0.push_back(g0_sp(b1, s, b2, s, k, w + off));
+  return g0;
 }
 
 // Owning version of g4acc_geometry (everything the reference constructor reads from
@@ -115,14 +164,7 @@ public:
     for (int s = 0; s < 2; ++s) {
       const auto& conf = configs[s];
       const int n = static_cast<int>(conf.size());
-      tau_[s].resize(n);
-      site_[s].resize(n);
-      band_[s].resize(n);
-      for (int i = 0; i < n; ++i) {
-        tau_[s][i] = conf[i].get_tau();
-        band_[s][i] = conf[i].get_right_band();
-        site_[s][i] = conf[i].get_right_site();
-      }
+      extract_configuration(conf, tau_[s], site_[s], band_[s]);
       w.n[s] = n;
       w.m[s] = n ? M_pair[s].ptr() : nullptr;
       w.ld[s] = n ? M_pair[s].leadingDimension() : 1;

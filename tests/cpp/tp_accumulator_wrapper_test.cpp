@@ -22,6 +22,8 @@ struct Vertex {
   double get_tau() const { return tau; }
   int get_right_band() const { return band; }
   int get_right_site() const { return site; }
+  int get_left_band() const { return band; }
+  int get_left_site() const { return site; }
 };
 
 struct Matrix {
