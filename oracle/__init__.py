@@ -63,12 +63,32 @@ def lib():
         L.tpo_g_size.argtypes = [vp]
         L.tpo_sum_to.restype = ctypes.c_int
         L.tpo_sum_to.argtypes = [vp, vp]
+        L.tpo_keep_intermediate.argtypes = [vp, ctypes.c_int]
+        L.tpo_use_blas.restype = ctypes.c_int
+        L.tpo_use_blas.argtypes = [ctypes.c_char_p]
+        L.tpo_blas_name.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
 
 def _ptr(a, t):
     return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+BLAS_CANDIDATES = ("/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so", "libopenblas.so")
+
+
+def use_blas(enable: bool = True) -> str:
+    """Switches the oracle's complex GEMMs to BLAS zgemm_ (the first of BLAS_CANDIDATES that
+    loads, single-threaded) or back to its own loop; returns the name of the GEMM in use."""
+    L = lib()
+    if not enable:
+        L.tpo_use_blas(None)
+    else:
+        for path in BLAS_CANDIDATES:
+            if L.tpo_use_blas(path.encode()):
+                break
+    return L.tpo_blas_name().decode()
 
 
 class OracleAccumulator:
@@ -109,6 +129,10 @@ class OracleAccumulator:
 
     def resetAccumulation(self):
         lib().tpo_reset(self._h)
+
+    def keep_intermediate(self, keep: bool):
+        """False: the M_r_r_w_w copy for get_Mrrww is not kept (timing runs)."""
+        lib().tpo_keep_intermediate(self._h, 1 if keep else 0)
 
     def _args(self, meas):
         keep = []

@@ -4,11 +4,16 @@
 // benchmark. Nothing in the product loads it.
 //
 // Each function restates the reference file:line given in its comment, in the same loop order,
-// in double precision (the reference's default TpAccumulationPrecision). It uses its own complex
-// GEMM instead of BLAS zgemm_. The reference's "cpu w_ex" stdout print inside the w_ex loop
+// in double precision (the reference's default TpAccumulationPrecision). Its complex GEMMs use
+// its own loop by default (the checker needs no external library); tpo_use_blas() switches them
+// to BLAS zgemm_ from a shared library loaded at run time (the MKL runtime of the image), which
+// is what the reference calls (tp/ndft/cached_ndft_cpu.hpp:214,217; space_transform_2D.hpp:80-81),
+// for the CPU baseline. The reference's "cpu w_ex" stdout print inside the w_ex loop
 // (tp_accumulator_cpu.hpp:460,500) is left out; everything else, including the per-call
 // allocation of M_r_r_w_w (tp_accumulator_cpu.hpp:261-262), is kept.
 #include "tp_oracle.h"
+
+#include <dlfcn.h>
 
 #include <algorithm>
 #include <cmath>
@@ -22,10 +27,21 @@ namespace {
 
 using cd = std::complex<double>;
 
+// BLAS zgemm_ (Fortran ABI, include/dca/linalg/blas/blas3.hpp:29), when loaded.
+using zgemm_fn = void (*)(const char*, const char*, const int*, const int*, const int*, const cd*,
+                          const cd*, const int*, const cd*, const int*, const cd*, cd*, const int*);
+zgemm_fn g_blas_zgemm = nullptr;
+std::string g_blas_name = "own loop zgemm";
+
 // Column-major complex GEMM, C = alpha * A * op(B) + beta * C, op(B) = B ('N') or B^H ('C').
 // Stands in for BLAS zgemm_ as called by linalg::matrixop::gemm (include/dca/linalg/matrixop.hpp).
 void zgemm(char tb, int m, int n, int k, cd alpha, const cd* A, int lda, const cd* B, int ldb,
            cd beta, cd* C, int ldc) {
+  if (g_blas_zgemm) {
+    const char ta = 'N';
+    g_blas_zgemm(&ta, &tb, &m, &n, &k, &alpha, A, &lda, B, &ldb, &beta, C, &ldc);
+    return;
+  }
   for (int j = 0; j < n; ++j) {
     cd* c = C + static_cast<size_t>(ldc) * j;
     if (beta == cd(0))
@@ -83,6 +99,7 @@ struct tpo_acc {
   std::vector<cd> Mrrww;      // last M_r_r_w_w, (r1, r2, b1, b2, s, w1E, w2E)
   std::vector<std::vector<cd>> G4;
   CMat Ga, Gb, G0M;           // band-block work spaces (tp_accumulator_cpu.hpp:208)
+  bool keep_mrrww = true;     // copy M_r_r_w_w for tpo_get_Mrrww (a test hook; off for timing)
 
   size_t g0i(int b1, int b2, int s, int k, int w) const {
     return b1 + nb * (b2 + nb * (s + 2 * (k + static_cast<size_t>(nc) * w)));
@@ -432,7 +449,8 @@ struct tpo_acc {
         continue;
       flops += ndft(s, n[s], m[s], ld[s], tau[s], site[s], band[s], M_r_r_w_w);
     }
-    Mrrww = M_r_r_w_w;
+    if (keep_mrrww)
+      Mrrww = M_r_r_w_w;
     space_transform(M_r_r_w_w);
     compute_G();
     flops += 1e-9 * 8 * std::pow(nb, 3) * nwE * nwE * nc * nc * 2;
@@ -498,6 +516,37 @@ tpo_acc* tpo_create(const tpo_geometry* g, char* err, int errlen) {
 }
 
 void tpo_destroy(tpo_acc* a) { delete a; }
+
+void tpo_keep_intermediate(tpo_acc* a, int keep) {
+  a->keep_mrrww = keep != 0;
+  if (!keep)
+    std::vector<cd>().swap(a->Mrrww);
+}
+
+int tpo_use_blas(const char* path) {
+  if (!path) {
+    g_blas_zgemm = nullptr;
+    g_blas_name = "own loop zgemm";
+    return 0;
+  }
+  void* lib = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!lib)
+    return 0;
+  auto fn = reinterpret_cast<zgemm_fn>(dlsym(lib, "zgemm_"));
+  if (!fn)
+    return 0;
+  // One accumulator per thread (the reference's threading model): BLAS itself single-threaded
+  // (MKL's sequential threading layer, MKL_THREADING_SEQUENTIAL = 1; OpenBLAS with one thread).
+  if (auto layer = reinterpret_cast<int (*)(int)>(dlsym(lib, "MKL_Set_Threading_Layer")))
+    layer(1);
+  else if (auto oset = reinterpret_cast<void (*)(int)>(dlsym(lib, "openblas_set_num_threads")))
+    oset(1);
+  g_blas_zgemm = fn;
+  g_blas_name = std::string("zgemm_ from ") + path;
+  return 1;
+}
+
+const char* tpo_blas_name() { return g_blas_name.c_str(); }
 
 // resetAccumulation (tp_accumulator_cpu.hpp:224-229).
 void tpo_reset(tpo_acc* a) {

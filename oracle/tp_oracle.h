@@ -53,6 +53,13 @@ typedef struct tpo_acc tpo_acc;
 /* Returns NULL and writes a message to err (if non-NULL, size errlen) on failure. */
 tpo_acc* tpo_create(const tpo_geometry* g, char* err, int errlen);
 void tpo_destroy(tpo_acc* a);
+/* keep = 0: do not copy the M_r_r_w_w intermediate for tpo_get_Mrrww (timing runs). */
+void tpo_keep_intermediate(tpo_acc* a, int keep);
+/* Complex GEMMs through BLAS zgemm_ from the shared library at path (single-threaded), or the
+ * oracle's own loop for path == NULL. Returns 1 if zgemm_ was loaded. Process-wide. */
+int tpo_use_blas(const char* path);
+/* Which GEMM the oracle uses. */
+const char* tpo_blas_name(void);
 void tpo_reset(tpo_acc* a);
 
 /* One measurement. n[s] vertices for spin s with times tau[s], sites site[s], bands band[s];
