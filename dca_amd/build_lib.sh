@@ -21,3 +21,13 @@ for p in "${pids[@]}"; do wait "$p"; done
 /opt/rocm/bin/hipcc --offload-arch=gfx950 -shared -fPIC $(printf "%s.o " "${SRC[@]/#/$OBJ/}") -o "$OUT.tmp" 2> >(filter)
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"
+# G4 files in the reference's HDF5 format (host C over the image's HDF5 1.10 C library).
+HDF5=/opt/conda
+if [ -f "$HDF5/include/hdf5.h" ] && [ -f "$HDF5/lib/libhdf5.so" ]; then
+  gcc -O2 -std=c11 -fPIC -shared -Wall -I "$HDF5/include" "$HERE/csrc/g4io.c" -L "$HDF5/lib" -lhdf5 \
+      -Wl,-rpath,"$HDF5/lib" -o "$HERE/libg4acc_io.so.tmp"
+  mv "$HERE/libg4acc_io.so.tmp" "$HERE/libg4acc_io.so"
+  echo "built $HERE/libg4acc_io.so"
+else
+  echo "HDF5 C library not found under $HDF5: libg4acc_io.so not built" >&2
+fi
