@@ -79,6 +79,7 @@ class Cluster:
     add: np.ndarray    # (nc, nc) int, add[i, j] = index(k_i + k_j)
     sub: np.ndarray    # (nc, nc) int, sub[i, j] = index(k_j - k_i)
     dft: np.ndarray    # (nc, nc) complex, dft[k, r] = exp(i k.r)
+    rsub: np.ndarray | None = None  # (nc, nc) int, rsub[a, b] = index(r_b - r_a), sites folded
 
     @property
     def nc(self) -> int:
@@ -141,7 +142,21 @@ def cluster_2d(a1: Sequence[int], a2: Sequence[int]) -> Cluster:
             add[i, j] = _index_of(k, _fold_k(k[i] + k[j]))
             sub[i, j] = _index_of(k, _fold_k(k[j] - k[i]))
     dft = np.exp(1j * (k @ r.T))
-    return Cluster(r=r, k=k, add=add, sub=sub, dft=dft)
+    # Site differences folded into the supercell (the cluster domain's subtract for sites,
+    # cluster_domain_initializer.hpp:688-770): rsub[a, b] = index(r_b - r_a).
+    def site_index(v):
+        f = Ainv @ v
+        f = f - np.floor(f + 1e-9)
+        rep = np.round(A @ f)
+        hit = np.nonzero(np.all(np.abs(r - rep[None, :]) < 1e-6, axis=1))[0]
+        if len(hit) != 1:
+            raise ValueError(f"site {v} not found in the cluster")
+        return int(hit[0])
+    rsub = np.empty((nc, nc), np.int32)
+    for a in range(nc):
+        for b in range(nc):
+            rsub[a, b] = site_index(r[b] - r[a])
+    return Cluster(r=r, k=k, add=add, sub=sub, dft=dft, rsub=rsub)
 
 
 def square_cluster(L: int) -> Cluster:
