@@ -75,6 +75,9 @@ struct SlotDesc {
   int n;
   int ld;
   double sign;         // the walker's sign (same for both spins)
+  // Non-null: m holds the walker's N matrix and M = diag(expv - 1) N is formed where M is read
+  // (CtauxWalker::computeM, ctaux_walker.hpp:1889-1916); expv[i] = exp(V) of vertex i.
+  const double* expv;
 };
 
 // Static geometry passed by value to kernels.

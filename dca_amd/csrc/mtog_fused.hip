@@ -16,16 +16,26 @@
 
 #include "g4acc_common.hpp"
 #include "g4acc_kernels.hpp"
+#include "site_fft.hpp"
 
 namespace g4acc {
 
-template <typename Real, int NC>
+// Side of the square cluster whose site DFT factorises into two L-point passes (0: none).
+template <int NC>
+constexpr int grid_side() {
+  return NC == 4 ? 2 : NC == 16 ? 4 : NC == 36 ? 6 : NC == 64 ? 8 : 0;
+}
+
+// SEP: the sites are labelled x + L y (square L x L cluster) and the left site transform runs
+// as an in-register FFT (site_fft.hpp); output kx + L ky goes to the caller's momentum kmap[.].
+template <typename Real, int NC, bool SEP>
 __global__ void __launch_bounds__(256) mtog_left_fused(const SlotDesc* __restrict__ desc, Geo g,
                                                        int n_max, int slot0,
                                                        const int* __restrict__ perm,
                                                        const int* __restrict__ off,
                                                        const Cplx<Real>* __restrict__ T,
                                                        const Cplx<Real>* __restrict__ dft,
+                                                       const int* __restrict__ kmap,
                                                        Cplx<Real>* __restrict__ A) {
   using C = Cplx<Real>;
   const int slot = slot0 + blockIdx.z;
@@ -38,12 +48,14 @@ __global__ void __launch_bounds__(256) mtog_left_fused(const SlotDesc* __restric
   const int* O = off + static_cast<size_t>(slot) * (NC + 1);
   const C* Ts = T + static_cast<size_t>(slot) * g.nwE * n_max;
   const double* mcol = d.m + static_cast<size_t>(d.ld) * P[jj];
+  const double* xv = d.expv;  // N and exp(V): M_ij = (exp(V_i) - 1) N_ij, formed in FP64
   C b[NC];
 #pragma unroll
   for (int r = 0; r < NC; ++r) {
     C acc{0, 0};
     for (int ii = O[r]; ii < O[r + 1]; ++ii) {
-      const Real m = static_cast<Real>(mcol[P[ii]]);
+      const int i = P[ii];
+      const Real m = static_cast<Real>(xv ? (xv[i] - 1.0) * mcol[i] : mcol[i]);
       const C tw = Ts[w + static_cast<size_t>(g.nwE) * ii];
       acc.x = fma(tw.x, m, acc.x);
       acc.y = fma(tw.y, m, acc.y);
@@ -51,86 +63,25 @@ __global__ void __launch_bounds__(256) mtog_left_fused(const SlotDesc* __restric
     b[r] = acc;
   }
   C* Ac = A + static_cast<size_t>(blockIdx.z) * g.D * n_max + static_cast<size_t>(g.D) * jj + NC * w;
+  if constexpr (SEP) {
+    constexpr int L = grid_side<NC>();
+    site_transform_2d<L, 1, Real>(b);
+#pragma unroll
+    for (int kk = 0; kk < NC; ++kk)
+      Ac[kmap[kk]] = b[kk];
+  }
+  else {
 #pragma unroll 4
-  for (int k = 0; k < NC; ++k) {
-    C acc{0, 0};
+    for (int k = 0; k < NC; ++k) {
+      C acc{0, 0};
 #pragma unroll
-    for (int r = 0; r < NC; ++r)
-      cfma(acc, dft[k + NC * r], b[r]);
-    Ac[k] = acc;
-  }
-}
-
-// Side of the square cluster whose site DFT factorises into two L-point passes (0: none).
-template <int NC>
-constexpr int grid_side() {
-  return NC == 4 ? 2 : NC == 16 ? 4 : NC == 36 ? 6 : NC == 64 ? 8 : 0;
-}
-
-// cos(2 pi m / L) and sin(2 pi m / L) for the sides above (0 <= m < L).
-__host__ __device__ constexpr double unit_cos(int L, int m) {
-  return L == 6 ? (m == 0 ? 1.0 : m == 3 ? -1.0 : (m == 1 || m == 5) ? 0.5 : -0.5)
-                : (m == 0 ? 1.0 : m == 4 ? -1.0 : (m == 2 || m == 6) ? 0.0
-                                                : (m == 1 || m == 7) ? 0.70710678118654752440
-                                                                     : -0.70710678118654752440);
-}
-__host__ __device__ constexpr double unit_sin(int L, int m) {
-  return L == 6 ? ((m == 0 || m == 3) ? 0.0 : (m == 1 || m == 2) ? 0.86602540378443864676
-                                                                 : -0.86602540378443864676)
-                : ((m == 0 || m == 4) ? 0.0 : m == 2 ? 1.0 : m == 6 ? -1.0
-                                         : (m == 1 || m == 3) ? 0.70710678118654752440
-                                                              : -0.70710678118654752440);
-}
-
-// v * exp(-2 pi i m / L). Quarter turns are exact swaps and negations; the loops that call this
-// are fully unrolled, so m and the branch are compile-time constants.
-template <int L, typename Real>
-__device__ __forceinline__ Cplx<Real> rot(Cplx<Real> v, int m) {
-  m %= L;
-  if ((4 * m) % L == 0) {
-    const int q = (4 * m) / L;
-    return q == 0 ? v : q == 1 ? Cplx<Real>{v.y, -v.x} : q == 2 ? Cplx<Real>{-v.x, -v.y} : Cplx<Real>{-v.y, v.x};
-  }
-  const Real c = static_cast<Real>(unit_cos(L, m)), sn = static_cast<Real>(unit_sin(L, m));
-  return Cplx<Real>{v.x * c + v.y * sn, v.y * c - v.x * sn};
-}
-
-// In place: v[kx + L ky] <- sum_{x,y} v[x + L y] exp(-2 pi i (kx x + ky y) / L), as two passes of
-// L-point sums (2 L^3 rotations instead of L^4 products for the dense site DFT).
-template <int L, typename Real>
-__device__ __forceinline__ void site_dft_2d(Cplx<Real> (&v)[L * L]) {
-  using C = Cplx<Real>;
-#pragma unroll
-  for (int x = 0; x < L; ++x) {
-    C col[L];
-#pragma unroll
-    for (int y = 0; y < L; ++y)
-      col[y] = v[x + L * y];
-#pragma unroll
-    for (int ky = 0; ky < L; ++ky) {
-      C acc = col[0];
-#pragma unroll
-      for (int y = 1; y < L; ++y)
-        acc = cadd(acc, rot<L, Real>(col[y], ky * y));
-      v[x + L * ky] = acc;
-    }
-  }
-#pragma unroll
-  for (int ky = 0; ky < L; ++ky) {
-    C row[L];
-#pragma unroll
-    for (int x = 0; x < L; ++x)
-      row[x] = v[x + L * ky];
-#pragma unroll
-    for (int kx = 0; kx < L; ++kx) {
-      C acc = row[0];
-#pragma unroll
-      for (int x = 1; x < L; ++x)
-        acc = cadd(acc, rot<L, Real>(row[x], kx * x));
-      v[kx + L * ky] = acc;
+      for (int r = 0; r < NC; ++r)
+        cfma(acc, dft[k + NC * r], b[r]);
+      Ac[k] = acc;
     }
   }
 }
+
 
 // Per chunk of JV sorted vertices the workgroup stages in LDS the A rows of its 64 p1 (all four
 // waves read the same rows) and the phases of its 4 TW frequencies; both are loaded
@@ -255,7 +206,7 @@ __global__ void __launch_bounds__(256) mtog_right_fused(const SlotDesc* __restri
         const int w2 = w2b + tt;
         if (w2 >= g.nwE)
           break;
-        site_dft_2d<L, Real>(e[tt]);
+        site_transform_2d<L, -1, Real>(e[tt]);
 #pragma unroll
         for (int kk = 0; kk < NC; ++kk) {
           const int k2 = Kl[kk];
@@ -538,7 +489,7 @@ __global__ void __launch_bounds__(256) mtog_right_mfma(const SlotDesc* __restric
       for (int r = 0; r < NC; ++r)
         e[r] = C{xr[r][ii], xi[r][ii]};
       if constexpr (L > 0)
-        site_dft_2d<L, float>(e);
+        site_transform_2d<L, -1, float>(e);
       else {
         C f[NC];
 #pragma unroll
@@ -575,21 +526,27 @@ namespace {
 
 template <typename Real, int NC, int TW>
 void launch_fused_nc(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st) {
-  // The separable site DFT is instantiated for the 2x2 and 4x4 clusters; for 6x6 and 8x8 the
-  // fully unrolled passes cost more registers than the dense DFT saves.
-  constexpr bool kSep = NC == 4 || NC == 16;
+  // The in-register FFT site transforms are instantiated for the 2x2, 4x4 and 8x8 tori (radix 2);
+  // 6x6 keeps the dense site sums (its two-pass L = 6 form needs more registers than it saves).
+  constexpr bool kSep = NC == 4 || NC == 16 || NC == 64;
   const Geo& g = a.g;
-  hipLaunchKernelGGL((mtog_left_fused<Real, NC>), dim3((g.nwE * a.n_max + 255) / 256, 1, ns), dim3(256), 0,
-                     st, a.desc, g, a.n_max, s0, a.perm, a.off, a.T, a.dft, a.A);
+  const bool sep = kSep && a.grid_L == grid_side<NC>() && a.kmap;
+  const dim3 lgrid((g.nwE * a.n_max + 255) / 256, 1, ns);
+  if (sep)
+    hipLaunchKernelGGL((mtog_left_fused<Real, NC, kSep>), lgrid, dim3(256), 0, st, a.desc, g, a.n_max, s0,
+                       a.perm, a.off, a.T, a.dft, a.kmap, a.A);
+  else
+    hipLaunchKernelGGL((mtog_left_fused<Real, NC, false>), lgrid, dim3(256), 0, st, a.desc, g, a.n_max, s0,
+                       a.perm, a.off, a.T, a.dft, nullptr, a.A);
   // Matrix-core right kernel (complex64): Nc <= 8, or 16 sites of a 4x4 torus (the dense
   // 16-point DFT epilogue would not fit in registers), with its step table in range.
-  const bool sep = kSep && a.grid_L == grid_side<NC>() && a.kmap;
   if constexpr (std::is_same<Real, float>::value && NC <= 16) {
     if (a.mfma && (NC <= 8 || sep) && a.n_max <= mtog_mfma_max_vertices(NC)) {
+      constexpr bool kSepM = NC == 4 || NC == 16;
       const int rows = NC == 16 ? 32 : 64;  // p1 rows per workgroup (two waves per tile for Nc = 16)
       const dim3 grid((g.D + rows - 1) / rows, 1, ns);  // each workgroup walks all frequency tiles
       if (sep)
-        hipLaunchKernelGGL((mtog_right_mfma<NC, kSep>), grid, dim3(256), 0, st, a.desc, g, a.n_max, s0, a.off,
+        hipLaunchKernelGGL((mtog_right_mfma<NC, kSepM>), grid, dim3(256), 0, st, a.desc, g, a.n_max, s0, a.off,
                            a.T, a.A, a.dft, a.bphase, a.G0, a.kmap, a.G);
       else if constexpr (NC <= 8)
         hipLaunchKernelGGL((mtog_right_mfma<NC, false>), grid, dim3(256), 0, st, a.desc, g, a.n_max, s0, a.off,

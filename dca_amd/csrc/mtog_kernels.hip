@@ -115,9 +115,11 @@ __global__ void __launch_bounds__(256) ndft_left_kernel(const SlotDesc* __restri
   const int* O = off + static_cast<size_t>(slot) * (g.n_orb + 1);
   const C* Ts = T + static_cast<size_t>(slot) * g.nwE * n_max;
   const double* mcol = d.m + static_cast<size_t>(d.ld) * P[jj];
+  const double* xv = d.expv;  // N and exp(V): M_ij = (exp(V_i) - 1) N_ij, formed in FP64
   C acc{0, 0};
   for (int ii = O[o]; ii < O[o + 1]; ++ii) {
-    const Real m = static_cast<Real>(mcol[P[ii]]);
+    const int i = P[ii];
+    const Real m = static_cast<Real>(xv ? (xv[i] - 1.0) * mcol[i] : mcol[i]);
     const C t = Ts[w + static_cast<size_t>(g.nwE) * ii];
     acc.x = fma(t.x, m, acc.x);
     acc.y = fma(t.y, m, acc.y);
@@ -255,8 +257,27 @@ __global__ void __launch_bounds__(256) dft_right_G_kernel(Geo g, int slot0,
     }
 }
 
+// M = diag(exp(V) - 1) N: the M matrix a CT-AUX walker hands to the accumulator
+// (CtauxWalker::computeM, ctaux_walker.hpp:1889-1916, which forms exp(V_i) - 1 per vertex and
+// scales the rows of N, src/linalg/lapack/multiply_diagonal_gpu.cu). One thread per entry, rows
+// fastest (coalesced columns).
+__global__ void __launch_bounds__(256) compute_m_kernel(int n, const double* __restrict__ exp_v,
+                                                        const double* __restrict__ N, int ldn,
+                                                        double* __restrict__ M, int ldm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n)
+    return;
+  M[i + static_cast<size_t>(ldm) * j] = (exp_v[i] - 1.0) * N[i + static_cast<size_t>(ldn) * j];
+}
+
 // ---------------------------------------------------------------------------------------------
 // Host launchers.
+
+void launch_compute_m(int n, const double* exp_v, const double* N, int ldn, double* M, int ldm,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(compute_m_kernel, dim3((n + 255) / 256, n), dim3(256), 0, st, n, exp_v, N, ldn, M, ldm);
+}
 
 void launch_sort(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, const int* site_map,
                  int* perm, int* off, int* err_flag, hipStream_t st) {

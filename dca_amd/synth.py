@@ -194,3 +194,19 @@ def tiny_workload(nb: int = 1, L: int = 2, n_freq: int = 4, beta: float = 2.0,
     g = geo.make_geometry(nb, cl, beta, n_freq, list(w_ex), kex, channels, zeeman=zeeman)
     lo, hi = n_range
     return Workload(seed_cfg, g, _uniform_n(lo, hi), signed, 1, hi, 4, "tiny")
+
+
+def walker_n_pair(meas: Measurement, rs: Stream, gamma: float = 0.6):
+    """CT-AUX-style hand-off for a measurement: per spin the vertices' exp(V) = exp(-gamma s_HS)
+    with a random auxiliary spin s_HS = +-1 (CV::exp_V, ctaux/structs/cv.hpp:13-15 with one
+    interaction strength), and the N matrix with M = diag(exp(V) - 1) N, i.e. N = M scaled back.
+    Returns (measurement with N in place of M, [exp_v spin 0, exp_v spin 1])."""
+    Ns, xvs = [], []
+    for s in range(2):
+        n = meas.n[s]
+        hs = np.where(rs.uniform(n) < 0.5, 1.0, -1.0)
+        xv = np.exp(-gamma * hs)
+        Ns.append(np.asfortranarray(meas.M[s] / (xv - 1.0)[:, None]) if n else meas.M[s])
+        xvs.append(xv)
+    return Measurement(n=list(meas.n), M=Ns, tau=meas.tau, site=meas.site, band=meas.band,
+                       sign=meas.sign), xvs

@@ -246,6 +246,14 @@ def g4_definition(geom: geo.Geometry, G: np.ndarray, channel: int, sign: float,
     return out
 
 
+def compute_m(N: np.ndarray, exp_v: np.ndarray) -> np.ndarray:
+    """The M a CT-AUX walker hands to the accumulator: M = diag(exp(V) - 1) N, row i scaled by
+    exp(V_i) - 1 in FP64 (CtauxWalker::computeM, ctaux_walker.hpp:1889-1916, with
+    multiplyDiagonalLeft, src/linalg/lapack/multiply_diagonal_gpu.cu:33-41)."""
+    N = np.asarray(N, np.float64)
+    return np.asfortranarray((np.asarray(exp_v, np.float64) - 1.0)[:, None] * N)
+
+
 def normalize_g4(g4: np.ndarray, beta: float, accumulated_sign: float) -> np.ndarray:
     """The solvers' G4 normalisation after the accumulation: G4 / (accumulated_sign beta^2)
     (ctaux_cluster_solver.hpp:523-527, ctint_cluster_solver.hpp:511-516), in FP64."""

@@ -54,6 +54,30 @@ def test_case_file_layout(tmp_path):
     assert p.stat().st_size == size
 
 
+def test_mirror_host_logic():
+    """The mirror's std::logic_error cases that precede any device call (CPU): non-density-density
+    vertices and too few single-particle frequencies (tests/cpp/host_logic_test.cpp)."""
+    exe = os.path.join(HERE, "cpp", "_build", "host_logic_test")
+    if not os.path.exists(exe):
+        subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(HERE, "..", "include"),
+                        os.path.join(HERE, "cpp", "host_logic_test.cpp"), "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "OK", out.stdout
+
+
+def test_site_fft_host():
+    """The register site transforms of the M -> G kernels (dca_amd/csrc/site_fft.hpp), run on
+    the CPU against the direct double sum (tests/cpp/site_fft_test.cpp)."""
+    exe = os.path.join(HERE, "cpp", "_build", "site_fft_test")
+    if not os.path.exists(exe):
+        os.makedirs(os.path.dirname(exe), exist_ok=True)
+        subprocess.run(["g++", "-std=c++17", "-O2", "-Wno-unknown-pragmas", "-D__HIP_PLATFORM_AMD__",
+                        "-I/opt/rocm/include", os.path.join(HERE, "cpp", "site_fft_test.cpp"), "-o", exe],
+                       check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "OK", out.stdout
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("nb,L,precision,tol", [(1, 4, 0, 1e-12), (1, 4, 1, 1e-6), (3, 2, 0, 1e-12)])
 def test_cpp_mirror_matches_oracle(oracle_lib, tmp_path, nb, L, precision, tol):

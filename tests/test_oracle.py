@@ -323,3 +323,16 @@ def test_normalize_g4():
     assert np.allclose(out * 16.0, g4, rtol=1e-15, atol=0)
     out = bf.normalize_g4(g4, 2.0, -3.0)
     assert np.allclose(out * -12.0, g4, rtol=1e-15, atol=0)
+
+
+def test_compute_m_is_row_scaling():
+    """M = diag(exp(V) - 1) N (ctaux_walker.hpp:1889-1916): row i scaled by exp(V_i) - 1, so a
+    unit N gives the diagonal and a transposed scaling (columns) differs."""
+    rs = synth.Stream(7)
+    xv = np.exp(rs.normal(5))
+    assert np.array_equal(bf.compute_m(np.eye(5), xv), np.diag(xv - 1.0))
+    N = rs.normal(25).reshape(5, 5)
+    M = bf.compute_m(N, xv)
+    for i in range(5):
+        assert np.array_equal(M[i], (xv[i] - 1.0) * N[i])
+    assert np.abs(M - N * (xv - 1.0)[None, :]).max() > 1e-3
