@@ -94,7 +94,14 @@ struct g4acc_handle {
   void* d_R = nullptr;
   int* d_r1_planes = nullptr;
   int n_r1 = 0, r1_slot[4] = {-1, -1, -1, -1};
-  std::vector<std::pair<int, hipGraphExec_t>> graphs;  // key: (walkers << 1) | stage
+  // Single-particle accumulator (g4acc_sp_enable): sorted by caller orbital, exact M_r_w sums.
+  bool sp = false;
+  bool sp_sqr = false;
+  int sp_nw = 0;
+  int *d_sp_perm = nullptr, *d_sp_off = nullptr, *d_rsub = nullptr;
+  double2 *d_sp_T = nullptr, *d_sp_C = nullptr, *d_sp_acc = nullptr, *d_sp_acc_sqr = nullptr;
+  size_t sp_entries = 0;  // per spin: nb^2 Nc n_w
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;  // key: (walkers << 2) | stage
   // stage timing (profiling hook)
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -250,8 +257,10 @@ int detect_grid(int nc, const double* dft, std::vector<int>& xy, std::vector<int
 
 // Resolves the walkers of one chunk into slot descriptors; host inputs are packed into the
 // pinned staging buffer and copied once. Returns the number of slots.
+// expv (or nullptr): per walker and spin the exp(V) values of the vertices; M is then N and
+// diag(exp(V) - 1) N is formed by the M -> G kernels. expv lives where M does.
 int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int memory,
-                  hipStream_t st) {
+                  hipStream_t st, const double* const* const* expv = nullptr) {
   const int nb = h->g.nb, nc = h->g.nc;
   // The pinned descriptor and staging buffers are reused: wait until the previous copies are done.
   if (h->staged_pending) {
@@ -266,6 +275,7 @@ int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int m
       const int n = wk.n[s];
       d.n = n;
       d.sign = wk.sign;
+      d.expv = nullptr;
       if (n == 0) {
         d.m = nullptr;
         d.tau = nullptr;
@@ -286,6 +296,18 @@ int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int m
       else {
         d.m = wk.m[s];
         d.ld = wk.ld[s];
+      }
+      if (expv) {
+        const double* xv = expv[w][s];
+        if (!xv)
+          return h->fail(G4ACC_INVALID_ARG, "exp_v missing for a non-empty spin sector");
+        if (!(memory & 1)) {
+          std::memcpy(h->h_stage_f + of, xv, sizeof(double) * n);
+          d.expv = h->d_stage_f + of;
+          of += n;
+        }
+        else
+          d.expv = xv;
       }
       if (!(memory & 2)) {  // configuration in host memory: validated and packed
         std::memcpy(h->h_stage_f + of, wk.tau[s], sizeof(double) * n);
@@ -433,7 +455,31 @@ void run_g4(g4acc_handle* h, int n_walkers) {
   }
 }
 
+void launch_sp(g4acc_handle* h, int n_walkers, hipStream_t st) {
+  const int n_slots = 2 * n_walkers;
+  launch_sort(h->d_desc, h->g, n_slots, h->max_vertices, nullptr, h->d_sp_perm, h->d_sp_off, h->d_err, st);
+  SpArgs a;
+  a.g = h->g;
+  a.desc = h->d_desc;
+  a.n_slots = n_slots;
+  a.n_max = h->max_vertices;
+  a.n_w = h->sp_nw;
+  a.perm = h->d_sp_perm;
+  a.off = h->d_sp_off;
+  a.T = h->d_sp_T;
+  a.C = h->d_sp_C;
+  a.rsub = h->d_rsub;
+  a.acc = h->d_sp_acc;
+  a.acc_sqr = h->d_sp_acc_sqr;
+  a.squared_too = h->sp_sqr;
+  launch_sp_accumulate(a, st);
+}
+
 void launch_stage(g4acc_handle* h, int n_walkers, int stage, hipStream_t st) {
+  if (stage == 2) {
+    launch_sp(h, n_walkers, st);
+    return;
+  }
   if (stage == 0) {
     if (h->precision == G4ACC_FP64)
       run_mtog<double>(h, 2 * n_walkers, st);
@@ -451,7 +497,7 @@ void launch_stage(g4acc_handle* h, int n_walkers, int stage, hipStream_t st) {
 // replayed; only the descriptor copy precedes each replay. The captured sequences hold kernel
 // launches only (no memset nodes).
 int run_stage_graph(g4acc_handle* h, int n_walkers, int stage, hipStream_t st) {
-  const int key = (n_walkers << 1) | stage;
+  const int key = (n_walkers << 2) | stage;
   hipGraphExec_t exec = nullptr;
   for (auto& e : h->graphs)
     if (e.first == key)
@@ -493,6 +539,8 @@ int run_batch(g4acc_handle* h, int n_walkers) {
   if (h->timing)
     G4_TRY(h->check(hipEventRecord(h->ev[2], h->stream), "hipEventRecord"));
   G4_TRY(run_stage(h, n_walkers, 1, h->stream));
+  if (h->sp)
+    G4_TRY(run_stage(h, n_walkers, 2, h->stream));
   if (h->timing) {
     G4_TRY(h->check(hipEventRecord(h->ev[3], h->stream), "hipEventRecord"));
     G4_TRY(h->check(hipEventSynchronize(h->ev[3]), "hipEventSynchronize"));
@@ -660,7 +708,7 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
   for (auto& p : h->d_g4)
     if ((rc = h->alloc(reinterpret_cast<double2**>(&p), h->g4_entries, "G4")))
       return bail(rc);
-  h->stage_f_cap = W2 * (NM * NM + NM);
+  h->stage_f_cap = W2 * (NM * NM + 2 * NM);  // M (or N), tau and exp(V) per slot
   h->stage_i_cap = W2 * 2 * NM;
   if ((rc = h->alloc(&h->d_stage_f, h->stage_f_cap, "stage_f")) ||
       (rc = h->alloc(&h->d_stage_i, h->stage_i_cap, "stage_i")))
@@ -769,12 +817,24 @@ int g4acc_reset(g4acc_handle* h) {
   h->queue.clear();
   for (auto* p : h->d_g4)
     launch_zero16(p, h->g4_entries, h->stream);
+  if (h->sp) {
+    launch_zero16(h->d_sp_acc, 2 * h->sp_entries, h->stream);
+    if (h->sp_sqr)
+      launch_zero16(h->d_sp_acc_sqr, 2 * h->sp_entries, h->stream);
+  }
   h->initialized = true;
   h->g4_present = true;
   return h->check(hipGetLastError(), "zero G4");
 }
 
-int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_walkers, int memory) {
+}  // extern "C"
+
+namespace {
+
+// accumulate_batch over walkers whose M is given as it is (exp_v == nullptr) or as N with the
+// vertices' exp(V) values (exp_v[2 i + s]).
+int accumulate_batch_impl(g4acc_handle* h, const g4acc_walker* walkers, const double* const* exp_v,
+                          int n_walkers, int memory) {
   if (!h)
     return G4ACC_INVALID_ARG;
   if (!h->initialized)
@@ -786,19 +846,51 @@ int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_w
     return G4ACC_OK;  // after sumTo the reference's G4 vector is empty: nothing is accumulated
   G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
   std::vector<const g4acc_walker*> todo;
+  std::vector<const double* const*> todo_x;
   todo.reserve(n_walkers);
   for (int i = 0; i < n_walkers; ++i) {
     G4_TRY(check_walker(h, walkers[i]));
     if (walkers[i].n[0] + walkers[i].n[1] == 0)
       continue;  // empty configuration: no-op (tp_accumulator_cpu.hpp:238-239)
     todo.push_back(&walkers[i]);
+    if (exp_v)
+      todo_x.push_back(exp_v + 2 * i);
   }
   for (size_t i0 = 0; i0 < todo.size(); i0 += h->max_walkers) {
     const int nw_ = static_cast<int>(std::min<size_t>(h->max_walkers, todo.size() - i0));
-    G4_TRY(stage_walkers(h, todo.data() + i0, nw_, memory, h->stream));
+    G4_TRY(stage_walkers(h, todo.data() + i0, nw_, memory, h->stream,
+                         exp_v ? todo_x.data() + i0 : nullptr));
     G4_TRY(run_batch(h, nw_));
   }
   return G4ACC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_walkers, int memory) {
+  return accumulate_batch_impl(h, walkers, nullptr, n_walkers, memory);
+}
+
+int g4acc_accumulate_batch_n(g4acc_handle* h, const g4acc_walker* walkers, const double* const* exp_v,
+                             int n_walkers, int memory) {
+  if (h && n_walkers > 0 && !exp_v)
+    return h->fail(G4ACC_INVALID_ARG, "exp_v is NULL");
+  return accumulate_batch_impl(h, walkers, exp_v, n_walkers, memory);
+}
+
+int g4acc_compute_m(g4acc_handle* h, int n, const double* exp_v, const double* N, int ldn, double* M,
+                    int ldm) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  if (n < 0 || (n > 0 && (!exp_v || !N || !M || ldn < n || ldm < n)))
+    return h->fail(G4ACC_INVALID_ARG, "compute_m: invalid arguments");
+  if (n == 0)
+    return G4ACC_OK;
+  G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  launch_compute_m(n, exp_v, N, ldn, M, ldm, h->stream);
+  return h->check(hipGetLastError(), "compute_m launch");
 }
 
 int g4acc_accumulate(g4acc_handle* h, const g4acc_walker* walker, int memory) {
@@ -986,6 +1078,13 @@ int g4acc_sum_to(g4acc_handle* src, g4acc_handle* dst) {
   G4_TRY(src->check(hipStreamSynchronize(src->stream), "hipStreamSynchronize"));
   for (size_t c = 0; c < src->channels.size(); ++c)
     launch_g4_add(dst->d_g4[c], src->d_g4[c], src->g4_entries, src->sum_mode, dst->stream);
+  // The single-particle sums are added and kept in the source (SpAccumulator::sumTo,
+  // sp_accumulator_gpu.hpp:231-236).
+  if (src->sp && dst->sp && src->sp_entries == dst->sp_entries) {
+    launch_g4_add(dst->d_sp_acc, src->d_sp_acc, 2 * src->sp_entries, G4ACC_SUM_FP64, dst->stream);
+    if (src->sp_sqr && dst->sp_sqr)
+      launch_g4_add(dst->d_sp_acc_sqr, src->d_sp_acc_sqr, 2 * src->sp_entries, G4ACC_SUM_FP64, dst->stream);
+  }
   G4_TRY(dst->check(hipStreamSynchronize(dst->stream), "sum_to"));
   src->g4_present = false;
   return G4ACC_OK;
@@ -1051,7 +1150,7 @@ int g4acc_accumulate_G(g4acc_handle* h, const double* G_ref, double sign) {
   int rc = h->check(hipMemcpyAsync(tmp, G_ref, sizeof(double2) * n, hipMemcpyHostToDevice, h->stream),
                     "copy G");
   if (rc == G4ACC_OK) {
-    h->h_desc[0] = SlotDesc{nullptr, nullptr, nullptr, nullptr, 1, 1, sign};
+    h->h_desc[0] = SlotDesc{nullptr, nullptr, nullptr, nullptr, 1, 1, sign, nullptr};
     h->h_desc[1] = h->h_desc[0];
     rc = h->check(hipMemcpyAsync(h->d_desc, h->h_desc, 2 * sizeof(SlotDesc), hipMemcpyHostToDevice,
                                  h->stream), "copy desc");
@@ -1130,6 +1229,77 @@ int g4acc_set_tuning(g4acc_handle* h, const char* key, int value) {
   for (auto& e : h->graphs)
     (void)hipGraphExecDestroy(e.second);
   h->graphs.clear();
+  return G4ACC_OK;
+}
+
+int g4acc_sp_enable(g4acc_handle* h, int n_sp_freq, const int* r_sub, int accumulate_m_sqr) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  if (h->sp)
+    return h->fail(G4ACC_INVALID_ARG, "the single-particle accumulator is already enabled");
+  if (n_sp_freq < 1 || !r_sub)
+    return h->fail(G4ACC_INVALID_ARG, "n_sp_freq >= 1 and r_sub are required");
+  const int nc = h->g.nc;
+  for (int i = 0; i < nc * nc; ++i)
+    if (r_sub[i] < 0 || r_sub[i] >= nc)
+      return h->fail(G4ACC_INVALID_ARG, "r_sub entry out of range");
+  G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  G4_TRY(quiesce(h));
+  const size_t W2 = 2 * static_cast<size_t>(h->max_walkers), NM = h->max_vertices;
+  const size_t n_w = 2 * static_cast<size_t>(n_sp_freq), n_orb = h->g.n_orb;
+  const size_t entries = static_cast<size_t>(h->g.nb) * h->g.nb * nc * n_w;
+  G4_TRY(h->alloc(&h->d_sp_perm, W2 * NM, "sp perm"));
+  G4_TRY(h->alloc(&h->d_sp_off, W2 * (n_orb + 1), "sp off"));
+  G4_TRY(h->alloc(&h->d_rsub, static_cast<size_t>(nc) * nc, "r_sub"));
+  G4_TRY(h->alloc(&h->d_sp_T, W2 * NM * n_w, "sp phases"));
+  G4_TRY(h->alloc(&h->d_sp_C, W2 * n_orb * n_orb * n_w, "sp orbital sums"));
+  G4_TRY(h->alloc(&h->d_sp_acc, 2 * entries, "sp M_r_w"));
+  if (accumulate_m_sqr)
+    G4_TRY(h->alloc(&h->d_sp_acc_sqr, 2 * entries, "sp M_r_w_sqr"));
+  G4_TRY(h->check(hipMemcpy(h->d_rsub, r_sub, sizeof(int) * nc * nc, hipMemcpyHostToDevice), "copy r_sub"));
+  h->sp_nw = static_cast<int>(n_w);
+  h->sp_entries = entries;
+  h->sp_sqr = accumulate_m_sqr != 0;
+  launch_zero16(h->d_sp_acc, 2 * entries, h->stream);
+  if (h->sp_sqr)
+    launch_zero16(h->d_sp_acc_sqr, 2 * entries, h->stream);
+  G4_TRY(h->check(hipStreamSynchronize(h->stream), "sp zero"));
+  h->sp = true;
+  for (auto& e : h->graphs)
+    (void)hipGraphExecDestroy(e.second);
+  h->graphs.clear();
+  return G4ACC_OK;
+}
+
+size_t g4acc_sp_size(const g4acc_handle* h) { return h && h->sp ? 2 * 2 * h->sp_entries : 0; }
+
+int g4acc_sp_get_m_r_w(g4acc_handle* h, int squared, double* out, size_t n_complex) {
+  if (!h)
+    return G4ACC_INVALID_ARG;
+  if (!h->sp)
+    return h->fail(G4ACC_NOT_INITIALIZED, "the single-particle accumulator is not enabled");
+  if (squared && !h->sp_sqr)
+    return h->fail(G4ACC_INVALID_ARG, "M_r_w_sqr was not requested (accumulate_m_sqr = 0)");
+  if (!out || n_complex < g4acc_sp_size(h))
+    return h->fail(G4ACC_INVALID_ARG, "output buffer too small");
+  G4_TRY(g4acc_sync(h));
+  const size_t E = h->sp_entries;
+  std::vector<double2> acc(2 * E);
+  G4_TRY(h->check(hipMemcpy(acc.data(), squared ? h->d_sp_acc_sqr : h->d_sp_acc, sizeof(double2) * 2 * E,
+                            hipMemcpyDeviceToHost), "copy M_r_w"));
+  // [s][(b1 + nb (b2 + nb r)) n_w + w] -> (b1, s, b2, s', r, w), zero for s != s'.
+  const int nb = h->g.nb, nc = h->g.nc, nw = h->sp_nw;
+  std::memset(out, 0, sizeof(double2) * g4acc_sp_size(h));
+  for (int s = 0; s < 2; ++s)
+    for (int r = 0; r < nc; ++r)
+      for (int b2 = 0; b2 < nb; ++b2)
+        for (int b1 = 0; b1 < nb; ++b1)
+          for (int w = 0; w < nw; ++w) {
+            const double2 v = acc[s * E + (b1 + nb * (b2 + static_cast<size_t>(nb) * r)) * nw + w];
+            const size_t o = b1 + nb * (s + 2 * (b2 + static_cast<size_t>(nb) * (s + 2 * (r + static_cast<size_t>(nc) * w))));
+            out[2 * o] = v.x;
+            out[2 * o + 1] = v.y;
+          }
   return G4ACC_OK;
 }
 

@@ -31,6 +31,10 @@ struct MtoGArgs {
   const int* kmap;
 };
 
+// M = diag(exp_v - 1) N for one n x n matrix (column-major, leading dimensions ldn, ldm).
+void launch_compute_m(int n, const double* exp_v, const double* N, int ldn, double* M, int ldm,
+                      hipStream_t st);
+
 // site_map (or nullptr): the internal label of each caller site (sites sorted by it).
 void launch_sort(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, const int* site_map,
                  int* perm, int* off, int* err_flag, hipStream_t st);
@@ -112,6 +116,23 @@ int tiled_pairs_per_thread(int nc);
 std::vector<int> diagonal_tile_order(int nw, int strip);
 // LDS of one tile workgroup with momentum groups of qg (staged blocks and rank-one factors).
 size_t tiled_lds_bytes(const Geo& g, bool need_y, bool need_z, size_t cplx_bytes, int qg);
+
+// Single-particle accumulator (sp_acc.hip): exact FP64 M_r_w of the batch's slots added to acc
+// (and M_ij^2's to acc_sqr when squared_too), acc laid out [s][(b1 + nb (b2 + nb r)) n_w + w].
+struct SpArgs {
+  Geo g;
+  const SlotDesc* desc;
+  int n_slots, n_max, n_w;     // n_w = 2 * single-particle frequencies
+  const int* perm;             // sort of the slots' vertices by caller orbital b + nb r
+  const int* off;
+  double2* T;                  // [n_slots][n_max][n_w] phases
+  double2* C;                  // [n_slots][n_orb][n_orb][n_w] orbital-pair sums
+  const int* rsub;             // [Nc][Nc], rsub[a + Nc b] = index(r_b - r_a)
+  double2* acc;
+  double2* acc_sqr;
+  bool squared_too;
+};
+void launch_sp_accumulate(const SpArgs& a, hipStream_t st);
 
 // dst += src over n complex entries of a G4 buffer in the given sum mode.
 void launch_g4_add(void* dst, const void* src, size_t n, int sum_mode, hipStream_t st);

@@ -34,7 +34,8 @@ EXPORTED = [
     "g4acc_stream", "g4acc_last_error", "g4acc_compute_G", "g4acc_g_size", "g4acc_accumulate_G",
     "g4acc_set_timing", "g4acc_get_timing", "g4acc_set_tuning", "g4acc_accumulate_queued",
     "g4acc_accumulate_single",
-    "g4acc_flush", "g4acc_sync_copy",
+    "g4acc_flush", "g4acc_sync_copy", "g4acc_accumulate_batch_n", "g4acc_compute_m",
+    "g4acc_sp_enable", "g4acc_sp_get_m_r_w", "g4acc_sp_size",
 ]
 
 
@@ -109,6 +110,9 @@ def load_library(path: str = LIB_PATH):
         "g4acc_sync": (i32, [vp]),
         "g4acc_get_g4": (i32, [vp, i32, vp, sz]),
         "g4acc_get_g4_normalized": (i32, [vp, i32, ctypes.c_double, vp, sz]),
+        "g4acc_sp_enable": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int), i32]),
+        "g4acc_sp_get_m_r_w": (i32, [vp, i32, vp, sz]),
+        "g4acc_sp_size": (sz, [vp]),
         "g4acc_sum_to": (i32, [vp, vp]),
         "g4acc_g4_size": (sz, [vp]),
         "g4acc_bytes": (sz, [vp]),
@@ -123,6 +127,8 @@ def load_library(path: str = LIB_PATH):
         "g4acc_accumulate_single": (i32, [vp, ctypes.POINTER(Walker_c), i32]),
         "g4acc_flush": (i32, [vp]),
         "g4acc_sync_copy": (i32, [vp]),
+        "g4acc_accumulate_batch_n": (i32, [vp, ctypes.POINTER(Walker_c), ctypes.POINTER(vp), i32, i32]),
+        "g4acc_compute_m": (i32, [vp, i32, vp, vp, i32, vp, i32]),
         "g4acc_get_timing": (i32, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_longlong)]),
     }
@@ -266,6 +272,25 @@ class TpAccumulator:
         self._check(self.lib.g4acc_accumulate_batch(self._h, arr, len(measurements), HOST))
         self._inflight.append(keep)
 
+    def accumulate_batch_n(self, measurements, exp_v) -> None:
+        """Batch of host measurements whose `M` fields hold the walkers' N matrices, with
+        exp_v[i][s] the exp(V) values of their vertices: M = diag(exp_v - 1) N is formed on the
+        device (CtauxWalker::computeM, ctaux_walker.hpp:1889-1916)."""
+        keep = _Keep()
+        arr = (Walker_c * len(measurements))(*[walker_struct(m, keep) for m in measurements])
+        xv = (ctypes.c_void_p * (2 * len(measurements)))()
+        for i, pair in enumerate(exp_v):
+            for s in range(2):
+                a = keep.add(np.ascontiguousarray(pair[s], np.float64))
+                xv[2 * i + s] = a.ctypes.data if a.size else None
+        self._check(self.lib.g4acc_accumulate_batch_n(self._h, arr, xv, len(measurements), HOST))
+        self._inflight.append(keep)
+
+    def compute_m(self, n: int, exp_v_ptr: int, n_ptr: int, ldn: int, m_ptr: int, ldm: int) -> None:
+        """M = diag(exp_v - 1) N on the device for device pointers (CtauxWalker::computeM), on the
+        handle's stream."""
+        self._check(self.lib.g4acc_compute_m(self._h, int(n), exp_v_ptr, n_ptr, int(ldn), m_ptr, int(ldm)))
+
     def accumulate_queued(self, meas, memory: int = HOST, walker: Walker_c | None = None) -> None:
         """accumulate(M_pair, configs, sign) through the handle's queue: the measurement is copied
         and runs in a batch of max_walkers (or at flush / sync / get_G4 / sumTo). Host
@@ -319,6 +344,31 @@ class TpAccumulator:
             res.append(out)
         self._inflight.clear()
         return res
+
+    def write_G4_hdf5(self, path: str, accumulated_sign: float | None = None):
+        """Writes every channel's G4 (normalised by accumulated_sign beta^2 when given) to `path`
+        in the reference's HDF5 format: /functions/G4_<channel> (dca_data.hpp:331-336,
+        hdf5_writer.hpp:244-262), through libg4acc_io.so."""
+        from . import g4io
+        g4 = self.get_G4() if accumulated_sign is None else self.get_G4_normalized(accumulated_sign)
+        g4io.write_g4(path, self.geom, g4)
+
+    def enable_sp(self, n_sp_freq: int, accumulate_m_sqr: bool = False):
+        """Adds the single-particle accumulator (SpAccumulator, sp_accumulator_gpu.hpp) to this
+        handle: every accumulate then also adds the measurement's M_r_w."""
+        rs = np.ascontiguousarray(self.geom.cluster.rsub.ravel(order="F"), np.int32)
+        self._check(self.lib.g4acc_sp_enable(self._h, int(n_sp_freq),
+                                             rs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                             1 if accumulate_m_sqr else 0))
+        self.n_sp_freq = int(n_sp_freq)
+
+    def get_sign_times_M_r_w(self, squared: bool = False) -> np.ndarray:
+        """M_r_w (or M_r_w_sqr) shaped (b1, s1, b2, s2, r, w) (Fortran order)."""
+        n = self.lib.g4acc_sp_size(self._h)
+        out = np.empty(n, np.complex128)
+        self._check(self.lib.g4acc_sp_get_m_r_w(self._h, 1 if squared else 0, out.ctypes.data, n))
+        g = self.geom
+        return out.reshape((g.nb, 2, g.nb, 2, g.nc, 2 * self.n_sp_freq), order="F")
 
     def sumTo(self, other: "TpAccumulator"):
         self._check(self.lib.g4acc_sum_to(self._h, other._h))

@@ -119,6 +119,19 @@ int g4acc_accumulate(g4acc_handle* h, const g4acc_walker* walker, int memory);
 /* Many walkers' measurements in one batched launch sequence; equivalent to calling
  * g4acc_accumulate for each walker. n_walkers may exceed max_walkers (processed in chunks). */
 int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_walkers, int memory);
+/* g4acc_accumulate_batch for walkers that hand over N and the vertices' exp(V) instead of M, as a
+ * CT-AUX walker holds them (CtauxWalker::computeM, ctaux_walker.hpp:1889-1916): walkers[i].m[s]
+ * points to N_s (leading dimension ld[s]) and exp_v[2 i + s] to the n[s] values exp(V_j) of its
+ * vertices (CV::exp_V), in the same memory as N. M_s = diag(exp_v - 1) N_s is formed in FP64 where
+ * the M -> G step reads it, so M never reaches memory; results equal g4acc_accumulate_batch on
+ * that M bit for bit. */
+int g4acc_accumulate_batch_n(g4acc_handle* h, const g4acc_walker* walkers, const double* const* exp_v,
+                             int n_walkers, int memory);
+/* CtauxWalker::computeM on the device (multiplyDiagonalLeft, src/linalg/lapack/multiply_diagonal_gpu.cu):
+ * M = diag(exp_v - 1) N for one n x n spin sector; exp_v, N and M are device pointers, column-major
+ * with leading dimensions ldn and ldm; runs on the handle's stream (g4acc_stream). */
+int g4acc_compute_m(g4acc_handle* h, int n, const double* exp_v, const double* N, int ldn, double* M,
+                    int ldm);
 /* accumulate(M_pair, configs, sign) with batching inside the handle, for callers that hand over
  * one measurement at a time (the reference launches each one on its own,
  * tp_accumulator_gpu.hpp:321-355). The measurement is copied into the handle's queue: host
@@ -169,6 +182,25 @@ size_t g4acc_g_size(const g4acc_handle* h);
  * in the reference layout (b1,b2,s,k1,k2,w1E,w2E), host memory, skipping M -> G. Same effect on
  * G4 as the reference's updateG4 loop over channels (tp_accumulator_cpu.hpp:248-249). */
 int g4acc_accumulate_G(g4acc_handle* h, const double* G, double sign);
+
+/* The single-particle accumulator beside the two-particle one: SpAccumulator<Parameters, GPU>
+ * (sp_accumulator_gpu.hpp; constructor :131-141, accumulate :165-190, finalize :215-229,
+ * sumTo :231-236). Once enabled, every accumulate call of the handle also adds
+ *   M_r_w(b1, s, b2, s, r, w) += sign / Nc sum_{i, j} M_s(i, j) exp(i w (tau_i - tau_j))
+ *                                 [b_i = b1, b_j = b2, r_i - r_j = r]
+ * over the 2 n_sp_freq single-particle Matsubara frequencies w = +-(2l + 1) pi / beta
+ * (frequency_domain.cpp:29-49), and its M_ij^2 counterpart when accumulate_m_sqr != 0. The
+ * reference approximates this sum with a delayed NFFT (dnfft_1d.hpp); here it is evaluated
+ * exactly in FP64 (the two agree to the NFFT's approximation, below 1e-9 from 64 frequencies on).
+ * r_sub[a + Nc b] = index(r_b - r_a) is the cluster's site subtraction. g4acc_reset clears it,
+ * g4acc_sum_to adds it like G4. */
+int g4acc_sp_enable(g4acc_handle* h, int n_sp_freq, const int* r_sub, int accumulate_m_sqr);
+/* get_sign_times_M_r_w() (squared = 0) or get_sign_times_M_r_w_sqr() (squared = 1) after
+ * finalize: n_complex >= g4acc_sp_size(h) complex doubles in the layout (b1, s1, b2, s2, r, w),
+ * first index fastest, zero for s1 != s2 (MFunction of sp_accumulator.hpp). */
+int g4acc_sp_get_m_r_w(g4acc_handle* h, int squared, double* out, size_t n_complex);
+/* Complex entries of M_r_w (0 when the single-particle accumulator is not enabled). */
+size_t g4acc_sp_size(const g4acc_handle* h);
 
 /* Profiling hook (the reference's Profiler scopes "accumulate"/"updateG4",
  * tp_accumulator_cpu.hpp:236,416): when enabled, every batch records HIP events around its
