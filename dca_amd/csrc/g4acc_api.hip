@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -25,6 +26,38 @@ namespace {
 thread_local std::string g_create_error;
 
 size_t workspace_budget_bytes() { return size_t(8) << 30; }
+
+}  // namespace
+
+// Workspace of g4acc_compute_G_noalloc. Each GPU holds its own copy at its own address until the
+// process ends. In host code this name is only a stand-in for the symbol: kernels and copies get
+// the per-device address that hipGetSymbolAddress returns (scratch_address below).
+static __device__ __align__(256) unsigned char scratch[G4ACC_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // by device ordinal; null until resolved on that device
+
+// The device address of `scratch` on `device`, which must be the current device. Resolved once per
+// device (the first g4acc_create there) and kept for the life of the process.
+int scratch_address(int device, void** out, std::string& err) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device >= static_cast<int>(g_scratch_ptr.size()))
+    g_scratch_ptr.resize(static_cast<size_t>(device) + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return e == hipSuccess ? G4ACC_DEVICE_ERROR : (e == hipErrorOutOfMemory ? G4ACC_OUT_OF_MEMORY
+                                                                                  : G4ACC_DEVICE_ERROR);
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return G4ACC_OK;
+}
 
 }  // namespace
 
@@ -92,6 +125,9 @@ struct g4acc_handle {
   // stored in d_R (complex64, one G4-sized plane per channel) and added by the tile kernel.
   int g4_rank1 = 0;
   void* d_R = nullptr;
+  // This device's address of the static workspace of g4acc_compute_G_noalloc.
+  unsigned char* d_scratch = nullptr;
+  std::string scratch_err;
   int* d_r1_planes = nullptr;
   int n_r1 = 0, r1_slot[4] = {-1, -1, -1, -1};
   // Single-particle accumulator (g4acc_sp_enable): sorted by caller orbital, exact M_r_w sums.
@@ -621,6 +657,14 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
   h->device = opt->device;
   if ((rc = h->check(hipSetDevice(h->device), "hipSetDevice")) != G4ACC_OK)
     return bail(rc);
+  {
+    // Only g4acc_compute_G_noalloc needs the address; if it cannot be resolved, that entry point
+    // reports why and everything else works as before.
+    void* p = nullptr;
+    if (scratch_address(h->device, &p, h->scratch_err) == G4ACC_OK)
+      h->d_scratch = static_cast<unsigned char*>(p);
+    (void)hipGetLastError();
+  }
   if ((rc = h->check(hipStreamCreate(&h->stream), "hipStreamCreate")) != G4ACC_OK)
     return bail(rc);
   if ((rc = h->check(hipEventCreateWithFlags(&h->staged, hipEventDisableTiming), "hipEventCreate")) != G4ACC_OK)
@@ -1131,6 +1175,112 @@ int g4acc_compute_G(g4acc_handle* h, const g4acc_walker* walker, int memory, dou
     rc = h->check(hipMemcpy(out, tmp, sizeof(double2) * n, hipMemcpyDeviceToHost), "copy G");
   (void)hipFree(tmp);
   return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Byte offsets of the intermediates of one walker's M -> G inside the static workspace.
+struct NoAllocLayout {
+  size_t desc, perm, off, T, B, A, E, G, total;
+};
+
+NoAllocLayout noalloc_layout(const g4acc_handle* h, int n_max) {
+  const size_t cb = h->cplx_bytes(), D = h->g.D, N = n_max;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  NoAllocLayout l;
+  size_t o = 0;
+  l.desc = o;
+  o += up(2 * sizeof(SlotDesc));
+  l.perm = o;
+  o += up(2 * N * sizeof(int));
+  l.off = o;
+  o += up(2 * (static_cast<size_t>(h->g.n_orb) + 1) * sizeof(int));
+  l.T = o;
+  o += up(2 * N * h->g.nwE * cb);
+  l.A = o;
+  o += up(2 * N * D * cb);
+  // The four-stage (multiband) form also keeps the time-transformed M and the left-transformed E.
+  l.B = l.E = 0;
+  if (!h->fused_mtog) {
+    l.B = o;
+    o += up(2 * N * D * cb);
+    l.E = o;
+    o += up(2 * D * D * cb);
+  }
+  l.G = o;
+  o += up(2 * D * D * cb);
+  l.total = o;
+  return l;
+}
+
+template <typename Real>
+void launch_mtog_noalloc(g4acc_handle* h, const NoAllocLayout& l, int n_max, double* G_dev) {
+  unsigned char* s = h->d_scratch;
+  int* perm = reinterpret_cast<int*>(s + l.perm);
+  int* off = reinterpret_cast<int*>(s + l.off);
+  MtoGArgs<Real> a = mtog_args<Real>(h, 2);
+  a.desc = reinterpret_cast<const SlotDesc*>(s + l.desc);
+  a.n_max = n_max;
+  a.chunk = 2;  // both spin sectors in one pass over the workspace
+  a.perm = perm;
+  a.off = off;
+  a.T = reinterpret_cast<Cplx<Real>*>(s + l.T);
+  a.A = reinterpret_cast<Cplx<Real>*>(s + l.A);
+  a.B = h->fused_mtog ? nullptr : reinterpret_cast<Cplx<Real>*>(s + l.B);
+  a.E = h->fused_mtog ? nullptr : reinterpret_cast<Cplx<Real>*>(s + l.E);
+  a.G = reinterpret_cast<Cplx<Real>*>(s + l.G);
+  launch_sort(a.desc, h->g, 2, n_max, h->d_site_map, perm, off, h->d_err, h->stream);
+  launch_mtog<Real>(a, h->stream);
+  launch_export_G<Real>(a.G, reinterpret_cast<double2*>(G_dev), h->g, h->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int g4acc_compute_G_noalloc(g4acc_handle* h, const g4acc_walker* walker, double* G_dev, size_t n_complex) {
+  if (!h || !walker || !G_dev)
+    return G4ACC_INVALID_ARG;
+  if (n_complex < g4acc_g_size(h))
+    return h->fail(G4ACC_INVALID_ARG, "output buffer too small");
+  if (!h->d_scratch)
+    return h->fail(G4ACC_DEVICE_ERROR, "the static workspace has no device address: " + h->scratch_err);
+  G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
+  G4_TRY(check_walker(h, *walker));
+  const int n_max = std::max(1, std::max(walker->n[0], walker->n[1]));
+  const NoAllocLayout l = noalloc_layout(h, n_max);
+  if (l.total > G4ACC_NOALLOC_SCRATCH_BYTES)
+    return h->fail(G4ACC_INVALID_ARG, "compute_G_noalloc: this measurement needs " + std::to_string(l.total) +
+                                          " B of workspace, more than the static " +
+                                          std::to_string(G4ACC_NOALLOC_SCRATCH_BYTES) + " B");
+  // The pinned descriptor buffer is reused: wait until its previous copy has been read. Measurements
+  // queued by g4acc_accumulate_queued stay queued (their copies are not touched here).
+  if (h->staged_pending) {
+    G4_TRY(h->check(hipEventSynchronize(h->staged), "hipEventSynchronize"));
+    h->staged_pending = false;
+  }
+  for (int s = 0; s < 2; ++s) {
+    SlotDesc& d = h->h_desc[s];
+    d.n = walker->n[s];
+    d.ld = walker->ld[s];
+    d.sign = walker->sign;
+    d.expv = nullptr;
+    d.m = d.n ? walker->m[s] : nullptr;
+    d.tau = d.n ? walker->tau[s] : nullptr;
+    d.site = d.n ? walker->site[s] : nullptr;
+    d.band = d.n ? walker->band[s] : nullptr;
+  }
+  G4_TRY(h->check(hipMemcpyAsync(h->d_scratch + l.desc, h->h_desc, 2 * sizeof(SlotDesc), hipMemcpyHostToDevice,
+                                 h->stream), "hipMemcpyAsync(desc)"));
+  G4_TRY(h->check(hipEventRecord(h->staged, h->stream), "hipEventRecord"));
+  h->staged_pending = true;
+  if (h->precision == G4ACC_FP64)
+    launch_mtog_noalloc<double>(h, l, n_max, G_dev);
+  else
+    launch_mtog_noalloc<float>(h, l, n_max, G_dev);
+  return h->check(hipGetLastError(), "compute_G_noalloc launch");
 }
 
 int g4acc_accumulate_G(g4acc_handle* h, const double* G_ref, double sign) {

@@ -35,7 +35,7 @@ EXPORTED = [
     "g4acc_set_timing", "g4acc_get_timing", "g4acc_set_tuning", "g4acc_accumulate_queued",
     "g4acc_accumulate_single",
     "g4acc_flush", "g4acc_sync_copy", "g4acc_accumulate_batch_n", "g4acc_compute_m",
-    "g4acc_sp_enable", "g4acc_sp_get_m_r_w", "g4acc_sp_size",
+    "g4acc_sp_enable", "g4acc_sp_get_m_r_w", "g4acc_sp_size", "g4acc_compute_G_noalloc",
 ]
 
 
@@ -119,6 +119,7 @@ def load_library(path: str = LIB_PATH):
         "g4acc_stream": (vp, [vp]),
         "g4acc_last_error": (ctypes.c_char_p, [vp]),
         "g4acc_compute_G": (i32, [vp, ctypes.POINTER(Walker_c), i32, vp, sz]),
+        "g4acc_compute_G_noalloc": (i32, [vp, ctypes.POINTER(Walker_c), vp, sz]),
         "g4acc_g_size": (sz, [vp]),
         "g4acc_accumulate_G": (i32, [vp, vp, ctypes.c_double]),
         "g4acc_set_timing": (i32, [vp, i32]),

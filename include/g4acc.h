@@ -178,6 +178,19 @@ int g4acc_compute_G(g4acc_handle* h, const g4acc_walker* walker, int memory, dou
                     size_t n_complex);
 /* Complex entries of G in the reference layout. */
 size_t g4acc_g_size(const g4acc_handle* h);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): g4acc_compute_G of one
+ * walker whose M and configuration are in device memory (G4ACC_DEVICE), with G written in the same
+ * layout (FP64 complex, n_complex >= g4acc_g_size(h)) to the caller's device buffer G_dev. The call
+ * allocates nothing: every intermediate (slot descriptors, vertex sort, phases, the left-transformed
+ * M, G in the handle's precision) lives in one static device array of
+ * G4ACC_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the process ends.
+ * Measurements whose intermediates exceed it are refused with G4ACC_INVALID_ARG. The handle's own
+ * workspaces and G4 are not touched. All calls on one GPU share that array, which keeps the last
+ * call's contents: calls that overlap on one GPU (two handles, two streams, two host threads) race.
+ * Runs on the handle's stream and returns without waiting; G_dev is complete after g4acc_sync. */
+#define G4ACC_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int g4acc_compute_G_noalloc(g4acc_handle* h, const g4acc_walker* walker, double* G_dev, size_t n_complex);
 /* Testing entry (the minimum slice of the path): accumulates one measurement whose G is given
  * in the reference layout (b1,b2,s,k1,k2,w1E,w2E), host memory, skipping M -> G. Same effect on
  * G4 as the reference's updateG4 loop over channels (tp_accumulator_cpu.hpp:248-249). */
