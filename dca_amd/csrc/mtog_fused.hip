@@ -555,9 +555,12 @@ void launch_fused_nc(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st) {
     }
   }
   const dim3 grid((g.D + 63) / 64, (g.nwE + 4 * TW - 1) / (4 * TW), ns);
-  if (sep)
-    hipLaunchKernelGGL((mtog_right_fused<Real, NC, TW, kSep>), grid, dim3(256), 0, st, a.desc, g, a.n_max, s0,
-                       a.off, a.T, a.A, a.dft, a.bphase, a.G0, a.kmap, a.G);
+  // The complex64 8x8 FFT epilogue needs 256 VGPRs (one wave per SIMD) against 162 for the dense
+  // 64-point DFT (three waves), and ran 2.3x slower; FP64 keeps the FFT (both forms at one wave).
+  constexpr bool kSepRight = kSep && (NC != 64 || std::is_same<Real, double>::value);
+  if (sep && kSepRight)
+    hipLaunchKernelGGL((mtog_right_fused<Real, NC, TW, kSepRight>), grid, dim3(256), 0, st, a.desc, g, a.n_max,
+                       s0, a.off, a.T, a.A, a.dft, a.bphase, a.G0, a.kmap, a.G);
   else
     hipLaunchKernelGGL((mtog_right_fused<Real, NC, TW, false>), grid, dim3(256), 0, st, a.desc, g, a.n_max, s0,
                        a.off, a.T, a.A, a.dft, a.bphase, a.G0, nullptr, a.G);
