@@ -6,7 +6,7 @@ HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 OUT="$HERE/libg4acc.so"
 OBJ="$HERE/_build"
 mkdir -p "$OBJ"
-SRC=(g4acc_api mtog_kernels mtog_fused g4_kernels g4_tiled g4_mfma g4_band g4_rank1 sp_acc)
+SRC=(g4acc_api mtog_kernels mtog_fused mtog_large g4_kernels g4_tiled g4_mfma g4_band g4_rank1 sp_acc)
 FLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Wall
        -Xclang -target-feature -Xclang -packed-fp32-ops)
 filter() { grep -v "packed-fp32-ops' is not a recognized feature" >&2 || true; }

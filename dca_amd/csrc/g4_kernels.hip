@@ -12,6 +12,14 @@
 // q-k = subtract(k,q), w+v -> w + v and v-w -> v + nw - 1 - w (compact indices, shifted by the
 // extension into the EXTENDED domain of G).
 //
+// Models that are not spin symmetric (SS = false; the spin is part of the band index) read spin
+// sector 0 only, with the whole sign (tp_accumulator_cpu.hpp:668-757):
+//   PP_UP_DOWN(:670-705)  s [G_0(b1,b3; k1,k2; w1,w2) G_0(b2,b4; q-k1,q-k2; v-w1,v-w2)
+//                            - G_0(b1,b4; k1,q-k2; w1,v-w2) G_0(b2,b3; q-k1,k2; v-w1,w2)]
+//   MAGNETIC  (:707-750)  s [G_0(b2,b1; k1,k1+q; w1,w1+v) G_0(b3,b4; k2+q,k2; w2+v,w2)
+//                            - G_0(b3,b1; k2+q,k1+q; w2+v,w1+v) G_0(b2,b4; k1,k2; w1,w2)]
+// The sign s is a complex phase for complex models (SignType, type_help.hpp:299-312).
+//
 // Each launch sums all walkers of the batch for an entry in FP64 registers and touches the entry
 // of G4 once (FP64 read-modify-write, or FP32 value + FP32 Kahan compensation).
 #include "g4acc_common.hpp"
@@ -58,7 +66,7 @@ __device__ __forceinline__ void g4_store(void* g4, size_t i, double2 acc, int su
   }
 }
 
-template <typename Real, int CH>
+template <typename Real, int CH, bool SS>
 __global__ void __launch_bounds__(256) g4_generic_kernel(G4Args<Real> a) {
   const Geo& g = a.g;
   const int nb = g.nb, nc = g.nc, nw = g.nw;
@@ -83,10 +91,25 @@ __global__ void __launch_bounds__(256) g4_generic_kernel(G4Args<Real> a) {
 
   double2 acc{0, 0};
   for (int m = 0; m < a.n_walkers; ++m) {
-    const double alpha = 0.5 * a.desc[2 * m].sign;
+    const double f = SS ? 0.5 : 1.0;
+    const double2 alpha{f * a.desc[2 * m].sign, f * a.desc[2 * m].sign_im};
     const int s0 = 2 * m, s1 = 2 * m + 1;
     double2 c{0, 0};
-    if constexpr (CH == PH_MAGNETIC || CH == PH_CHARGE) {
+    if constexpr (!SS) {
+      if constexpr (CH == PP_UP_DOWN) {
+        const int qk1 = a.k_sub[k1 + nc * q], qk2 = a.k_sub[k2 + nc * q];
+        const int w1r = nu + nw - 1 - w1 + ext, w2r = nu + nw - 1 - w2 + ext;
+        c = csub(cmul(L(s0, b1, b3, k1, k2, w1e, w2e), L(s0, b2, b4, qk1, qk2, w1r, w2r)),
+                 cmul(L(s0, b1, b4, k1, qk2, w1e, w2r), L(s0, b2, b3, qk1, k2, w1r, w2e)));
+      }
+      else {  // PH_MAGNETIC
+        const int kq1 = a.k_add[k1 + nc * q], kq2 = a.k_add[k2 + nc * q];
+        const int w1n = w1 + nu + ext, w2n = w2 + nu + ext;
+        c = csub(cmul(L(s0, b2, b1, k1, kq1, w1e, w1n), L(s0, b3, b4, kq2, k2, w2n, w2e)),
+                 cmul(L(s0, b3, b1, kq2, kq1, w2n, w1n), L(s0, b2, b4, k1, k2, w1e, w2e)));
+      }
+    }
+    else if constexpr (CH == PH_MAGNETIC || CH == PH_CHARGE) {
       const int kq1 = a.k_add[k1 + nc * q], kq2 = a.k_add[k2 + nc * q];
       const int w1n = w1 + nu + ext, w2n = w2 + nu + ext;
       double2 ga0 = L(s0, b2, b1, k1, kq1, w1e, w1n), ga1 = L(s1, b2, b1, k1, kq1, w1e, w1n);
@@ -128,8 +151,8 @@ __global__ void __launch_bounds__(256) g4_generic_kernel(G4Args<Real> a) {
       c.x = -c.x;
       c.y = -c.y;
     }
-    acc.x = fma(alpha, c.x, acc.x);
-    acc.y = fma(alpha, c.y, acc.y);
+    acc.x = fma(alpha.x, c.x, fma(-alpha.y, c.y, acc.x));
+    acc.y = fma(alpha.x, c.y, fma(alpha.y, c.x, acc.y));
   }
   g4_store(a.g4, static_cast<size_t>(blockIdx.y) * S + e, acc, a.sum_mode);
 }
@@ -139,18 +162,25 @@ void launch_g4(const G4Args<Real>& a, hipStream_t st) {
   const Geo& g = a.g;
   const size_t S = static_cast<size_t>(g.nb) * g.nb * g.nb * g.nb * g.nc * g.nw * g.nc * g.nw;
   dim3 grid(static_cast<unsigned>((S + 255) / 256), g.nq * g.nv);
+  if (!a.spin_symmetric) {  // the host admits only the pp and magnetic channels here
+    if (a.channel == PP_UP_DOWN)
+      hipLaunchKernelGGL((g4_generic_kernel<Real, PP_UP_DOWN, false>), grid, dim3(256), 0, st, a);
+    else if (a.channel == PH_MAGNETIC)
+      hipLaunchKernelGGL((g4_generic_kernel<Real, PH_MAGNETIC, false>), grid, dim3(256), 0, st, a);
+    return;
+  }
   switch (a.channel) {
     case PH_MAGNETIC:
-      hipLaunchKernelGGL((g4_generic_kernel<Real, PH_MAGNETIC>), grid, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((g4_generic_kernel<Real, PH_MAGNETIC, true>), grid, dim3(256), 0, st, a);
       break;
     case PH_CHARGE:
-      hipLaunchKernelGGL((g4_generic_kernel<Real, PH_CHARGE>), grid, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((g4_generic_kernel<Real, PH_CHARGE, true>), grid, dim3(256), 0, st, a);
       break;
     case PP_UP_DOWN:
-      hipLaunchKernelGGL((g4_generic_kernel<Real, PP_UP_DOWN>), grid, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((g4_generic_kernel<Real, PP_UP_DOWN, true>), grid, dim3(256), 0, st, a);
       break;
     case PH_TRANSVERSE:
-      hipLaunchKernelGGL((g4_generic_kernel<Real, PH_TRANSVERSE>), grid, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((g4_generic_kernel<Real, PH_TRANSVERSE, true>), grid, dim3(256), 0, st, a);
       break;
     default:
       break;

@@ -25,7 +25,7 @@ namespace {
 
 thread_local std::string g_create_error;
 
-size_t workspace_budget_bytes() { return size_t(8) << 30; }
+size_t workspace_budget_bytes() { return size_t(16) << 30; }
 
 }  // namespace
 
@@ -89,6 +89,12 @@ struct g4acc_handle {
   int *d_perm = nullptr, *d_off = nullptr, *d_err = nullptr;
   void *d_T = nullptr, *d_B = nullptr, *d_A = nullptr, *d_E = nullptr, *d_G = nullptr;
   void* d_UV = nullptr;            // rank-one factors of the tiled single-band G4 kernel
+  // Model (g4acc_create_model). The general model (any field off its default) runs the
+  // four-stage M -> G and the per-entry G4 kernels.
+  int spin_symmetric = 1, complex_m = 0, non_dd = 0;
+  bool general() const { return !spin_symmetric || complex_m || non_dd; }
+  int *d_perm2 = nullptr, *d_off2 = nullptr;  // column sort by left labels (non-density-density)
+  void* d_T2 = nullptr;                       // and its phases
   bool fused_mtog = false;         // register-fused single-band M -> G (no E workspace)
   bool use_tiled = false;
   bool use_band = false;           // LDS-tiled multiband G4 kernel (G4ACC_GENERIC_G4=1 disables)
@@ -119,6 +125,10 @@ struct g4acc_handle {
   bool use_graph = true;
   int mtog_tw = 0;                 // tuning: frequencies per thread of the fused M -> G kernel
   int mtog_mfma = 0;               // tuning: fused right M -> G kernel on the matrix cores
+  // 6x6 / 8x8 square clusters: matrix-core S3 into the E workspace plus the site transform
+  // kernel (mtog_large.hip); large_ok when E was sized for it.
+  int mtog_large = 0;
+  bool large_ok = false;
   int g4_qg = 0;                   // tuning: momenta per G4 workgroup (0: register limit)
   int g4_mfma = 0;                 // tuning: rank-one G4 terms on the matrix cores (FP32 G, Nc <= 16)
   // tuning: rank-one MAGNETIC / CHARGE terms of a batch as one matrix-core GEMM (g4_rank1.hip),
@@ -291,12 +301,61 @@ int detect_grid(int nc, const double* dft, std::vector<int>& xy, std::vector<int
   return 0;
 }
 
+// One walker's inputs as the staging reads them. The real walker (g4acc_walker), the CT-AUX
+// hand-off (N with exp(V), g4acc_accumulate_batch_n) and the general model's walker all map onto it.
+struct WalkerIn {
+  int n[2];
+  const double* m[2];
+  int ld[2];
+  const double* tau[2];
+  const int* site[2];
+  const int* band[2];
+  const int* site_l[2];    // nullptr: the vertices are density-density
+  const int* band_l[2];
+  bool n_form;             // m holds N and M = diag(exp(V) - 1) N with expv[s] = exp(V)
+  const double* expv[2];
+  double sign, sign_im;
+  bool m_cplx;
+};
+
+WalkerIn walker_in(const g4acc_walker& w, const double* const* exp_v = nullptr) {
+  WalkerIn x{};
+  for (int s = 0; s < 2; ++s) {
+    x.n[s] = w.n[s];
+    x.m[s] = w.m[s];
+    x.ld[s] = w.ld[s];
+    x.tau[s] = w.tau[s];
+    x.site[s] = w.site[s];
+    x.band[s] = w.band[s];
+    x.expv[s] = exp_v ? exp_v[s] : nullptr;
+  }
+  x.n_form = exp_v != nullptr;
+  x.sign = w.sign;
+  return x;
+}
+
+WalkerIn walker_in(const g4acc_handle* h, const g4acc_walker_general& w) {
+  WalkerIn x{};
+  for (int s = 0; s < 2; ++s) {
+    x.n[s] = w.n[s];
+    x.m[s] = w.m[s];
+    x.ld[s] = w.ld[s];
+    x.tau[s] = w.tau[s];
+    x.site[s] = w.site[s];
+    x.band[s] = w.band[s];
+    x.site_l[s] = h->non_dd ? w.site_l[s] : nullptr;
+    x.band_l[s] = h->non_dd ? w.band_l[s] : nullptr;
+  }
+  x.sign = w.sign[0];
+  x.sign_im = w.sign[1];
+  x.m_cplx = h->complex_m != 0;
+  return x;
+}
+
 // Resolves the walkers of one chunk into slot descriptors; host inputs are packed into the
-// pinned staging buffer and copied once. Returns the number of slots.
-// expv (or nullptr): per walker and spin the exp(V) values of the vertices; M is then N and
-// diag(exp(V) - 1) N is formed by the M -> G kernels. expv lives where M does.
-int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int memory,
-                  hipStream_t st, const double* const* const* expv = nullptr) {
+// pinned staging buffer and copied once. Returns the number of slots. exp(V) values live where M
+// does; left labels where the configuration does.
+int stage_walkers(g4acc_handle* h, const WalkerIn* ws, int nw_, int memory, hipStream_t st) {
   const int nb = h->g.nb, nc = h->g.nc;
   // The pinned descriptor and staging buffers are reused: wait until the previous copies are done.
   if (h->staged_pending) {
@@ -305,13 +364,16 @@ int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int m
   }
   size_t of = 0, oi = 0;
   for (int w = 0; w < nw_; ++w) {
-    const g4acc_walker& wk = *ws[w];
+    const WalkerIn& wk = ws[w];
     for (int s = 0; s < 2; ++s) {
       SlotDesc& d = h->h_desc[2 * w + s];
       const int n = wk.n[s];
       d.n = n;
       d.sign = wk.sign;
+      d.sign_im = wk.sign_im;
+      d.m_cplx = wk.m_cplx ? 1 : 0;
       d.expv = nullptr;
+      d.site_l = d.band_l = nullptr;
       if (n == 0) {
         d.m = nullptr;
         d.tau = nullptr;
@@ -320,21 +382,22 @@ int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int m
         d.ld = 1;
         continue;
       }
+      const size_t per = wk.m_cplx ? 2 : 1;  // doubles per element of M
       if (!(memory & 1)) {  // M in host memory: packed column by column
+        of += of & (per - 1);  // complex entries start on a 16-byte boundary
         double* fm = h->h_stage_f + of;
         for (int j = 0; j < n; ++j)
-          std::memcpy(fm + static_cast<size_t>(n) * j, wk.m[s] + static_cast<size_t>(wk.ld[s]) * j,
-                      sizeof(double) * n);
+          std::memcpy(fm + per * n * j, wk.m[s] + per * wk.ld[s] * j, sizeof(double) * per * n);
         d.m = h->d_stage_f + of;
         d.ld = n;
-        of += static_cast<size_t>(n) * n;
+        of += per * n * n;
       }
       else {
         d.m = wk.m[s];
         d.ld = wk.ld[s];
       }
-      if (expv) {
-        const double* xv = expv[w][s];
+      if (wk.n_form) {
+        const double* xv = wk.expv[s];
         if (!xv)
           return h->fail(G4ACC_INVALID_ARG, "exp_v missing for a non-empty spin sector");
         if (!(memory & 1)) {
@@ -345,6 +408,7 @@ int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int m
         else
           d.expv = xv;
       }
+      const bool left = wk.site_l[s] != nullptr;
       if (!(memory & 2)) {  // configuration in host memory: validated and packed
         std::memcpy(h->h_stage_f + of, wk.tau[s], sizeof(double) * n);
         int* im = h->h_stage_i + oi;
@@ -354,17 +418,30 @@ int stage_walkers(g4acc_handle* h, const g4acc_walker* const* ws, int nw_, int m
             return h->fail(G4ACC_INVALID_ARG, "vertex site or band index out of range");
           im[i] = r;
           im[n + i] = b;
+          if (left) {
+            const int rl = wk.site_l[s][i], bl = wk.band_l[s][i];
+            if (rl < 0 || rl >= nc || bl < 0 || bl >= nb)
+              return h->fail(G4ACC_INVALID_ARG, "vertex left site or band index out of range");
+            im[2 * n + i] = rl;
+            im[3 * n + i] = bl;
+          }
         }
         d.tau = h->d_stage_f + of;
         d.site = h->d_stage_i + oi;
         d.band = h->d_stage_i + oi + n;
+        if (left) {
+          d.site_l = h->d_stage_i + oi + 2 * n;
+          d.band_l = h->d_stage_i + oi + 3 * n;
+        }
         of += n;
-        oi += 2 * static_cast<size_t>(n);
+        oi += (left ? 4 : 2) * static_cast<size_t>(n);
       }
       else {
         d.tau = wk.tau[s];
         d.site = wk.site[s];
         d.band = wk.band[s];
+        d.site_l = wk.site_l[s];
+        d.band_l = wk.band_l[s];
       }
     }
   }
@@ -393,9 +470,13 @@ MtoGArgs<Real> mtog_args(g4acc_handle* h, int n_slots) {
   a.fused = h->fused_mtog;
   a.tw = h->mtog_tw;
   a.mfma = h->mtog_mfma;
+  a.large = h->mtog_large != 0;
   a.perm = h->d_perm;
   a.off = h->d_off;
   a.T = static_cast<Cplx<Real>*>(h->d_T);
+  a.perm2 = h->non_dd ? h->d_perm2 : h->d_perm;
+  a.off2 = h->non_dd ? h->d_off2 : h->d_off;
+  a.T2 = static_cast<Cplx<Real>*>(h->non_dd ? h->d_T2 : h->d_T);
   a.B = static_cast<Cplx<Real>*>(h->d_B);
   a.A = static_cast<Cplx<Real>*>(h->d_A);
   a.E = static_cast<Cplx<Real>*>(h->d_E);
@@ -411,6 +492,9 @@ MtoGArgs<Real> mtog_args(g4acc_handle* h, int n_slots) {
 template <typename Real>
 void run_mtog(g4acc_handle* h, int n_slots, hipStream_t st) {
   launch_sort(h->d_desc, h->g, n_slots, h->max_vertices, h->d_site_map, h->d_perm, h->d_off, h->d_err, st);
+  if (h->non_dd)
+    launch_sort(h->d_desc, h->g, n_slots, h->max_vertices, h->d_site_map, h->d_perm2, h->d_off2, h->d_err, st,
+                true);
   launch_mtog<Real>(mtog_args<Real>(h, n_slots), st);
 }
 
@@ -487,6 +571,7 @@ void run_g4(g4acc_handle* h, int n_walkers) {
     a.w_ex = h->d_wex;
     a.sum_mode = h->sum_mode;
     a.g4 = h->d_g4[c];
+    a.spin_symmetric = h->spin_symmetric != 0;
     launch_g4<Real>(a, h->stream);
   }
 }
@@ -597,7 +682,8 @@ int quiesce(g4acc_handle* h) {
   return G4ACC_OK;
 }
 
-int check_walker(g4acc_handle* h, const g4acc_walker& w) {
+// Real walkers carry a sign of +-1; the general model's any finite (complex) sign.
+int check_walker(g4acc_handle* h, const WalkerIn& w, bool general = false) {
   for (int s = 0; s < 2; ++s) {
     if (w.n[s] < 0 || w.n[s] > h->max_vertices)
       return h->fail(G4ACC_INVALID_ARG, "vertex count " + std::to_string(w.n[s]) +
@@ -605,17 +691,30 @@ int check_walker(g4acc_handle* h, const g4acc_walker& w) {
                                             std::to_string(h->max_vertices) + "]");
     if (w.n[s] > 0 && (!w.m[s] || !w.tau[s] || !w.site[s] || !w.band[s] || w.ld[s] < w.n[s]))
       return h->fail(G4ACC_INVALID_ARG, "walker arrays missing or ld < n");
+    if (w.n[s] > 0 && (!w.site_l[s]) != (!w.band_l[s]))
+      return h->fail(G4ACC_INVALID_ARG, "left sites and left bands must be given together");
   }
-  if (!(w.sign == 1.0 || w.sign == -1.0))
+  if (general) {
+    if (!std::isfinite(w.sign) || !std::isfinite(w.sign_im))
+      return h->fail(G4ACC_INVALID_ARG, "sign must be finite");
+  }
+  else if (!(w.sign == 1.0 || w.sign == -1.0))
     return h->fail(G4ACC_INVALID_ARG, "sign must be +1 or -1");
   return G4ACC_OK;
 }
+int check_walker(g4acc_handle* h, const g4acc_walker& w) { return check_walker(h, walker_in(w)); }
 
 }  // namespace
 
 extern "C" {
 
 int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handle** out) {
+  const g4acc_model model{1, 0, 0};
+  return g4acc_create_model(gm, opt, &model, out);
+}
+
+int g4acc_create_model(const g4acc_geometry* gm, const g4acc_options* opt, const g4acc_model* model,
+                       g4acc_handle** out) {
   g_create_error.clear();
   if (!out) {
     g_create_error = "out is NULL";
@@ -634,6 +733,19 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
     g_create_error = "invalid options";
     return G4ACC_INVALID_ARG;
   }
+  if (!model || (model->spin_symmetric != 0 && model->spin_symmetric != 1) ||
+      (model->complex_m != 0 && model->complex_m != 1) ||
+      (model->non_density_density != 0 && model->non_density_density != 1)) {
+    g_create_error = "invalid model (each field 0 or 1)";
+    return G4ACC_INVALID_ARG;
+  }
+  if (!model->spin_symmetric)
+    for (int i = 0; i < gm->n_channels; ++i)
+      if (gm->channels[i] != G4ACC_PARTICLE_PARTICLE_UP_DOWN && gm->channels[i] != G4ACC_PARTICLE_HOLE_MAGNETIC) {
+        // tp_accumulator_cpu.hpp:752-753
+        g_create_error = "Specified four point type not implemented.";
+        return G4ACC_UNSUPPORTED_CHANNEL;
+      }
   int n_dev = 0;
   if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev < 1) {
     (void)hipGetLastError();
@@ -655,6 +767,9 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
     return code;
   };
   h->device = opt->device;
+  h->spin_symmetric = model->spin_symmetric;
+  h->complex_m = model->complex_m;
+  h->non_dd = model->non_density_density;
   if ((rc = h->check(hipSetDevice(h->device), "hipSetDevice")) != G4ACC_OK)
     return bail(rc);
   {
@@ -701,8 +816,14 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
   const size_t cb = h->cplx_bytes();
   // Slots whose S1..S4 workspaces (B, A: D x N; E: D x D) exist at once. The fused single-band
   // kernels need only A.
-  h->fused_mtog = mtog_fused_covers(g) && getenv("G4ACC_UNFUSED_MTOG") == nullptr;
-  const size_t per_slot = h->fused_mtog ? cb * D * NM : cb * (2 * D * NM + D * D);
+  h->fused_mtog = mtog_fused_covers(g) && !h->general() && getenv("G4ACC_UNFUSED_MTOG") == nullptr;
+  // 6x6 and 8x8 tori: the right half runs as matrix-core S3 into E plus the site transform.
+  std::vector<int> xy, kxy_to_k;
+  if (getenv("G4ACC_DENSE_DFT") == nullptr)
+    h->grid_L = detect_grid(g.nc, gm->dft, xy, kxy_to_k);
+  h->large_ok = h->fused_mtog && (g.nc == 36 || g.nc == 64) && h->grid_L * h->grid_L == g.nc;
+  h->mtog_large = h->large_ok ? 1 : 0;
+  const size_t per_slot = h->fused_mtog ? cb * D * (NM + (h->large_ok ? D : 0)) : cb * (2 * D * NM + D * D);
   h->chunk = static_cast<int>(std::max<size_t>(1, std::min<size_t>(W2, workspace_budget_bytes() / per_slot)));
   const size_t CH = h->chunk;
 
@@ -716,11 +837,17 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_T), cb * W2 * NM * g.nwE, "T")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_B), h->fused_mtog ? 1 : cb * CH * NM * D, "B")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_A), cb * CH * NM * D, "A")) ||
-      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_E), h->fused_mtog ? 1 : cb * CH * D * D, "E")) ||
+      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_E),
+                     h->fused_mtog && !h->large_ok ? 1 : cb * CH * D * D, "E")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_G), cb * W2 * D * D, "G")) ||
       (rc = h->alloc(&h->d_desc, W2, "desc")))
     return bail(rc);
   if (gm->band_phase && (rc = h->alloc(&h->d_bphase, static_cast<size_t>(g.nb) * g.nc, "band_phase")))
+    return bail(rc);
+  // Non-density-density vertices: a second sort (columns of M by left labels) and its phases.
+  if (h->non_dd &&
+      ((rc = h->alloc(&h->d_perm2, W2 * NM, "perm2")) || (rc = h->alloc(&h->d_off2, W2 * (g.n_orb + 1), "off2")) ||
+       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_T2), cb * W2 * NM * g.nwE, "T2"))))
     return bail(rc);
   // The LDS-tiled G4 kernel covers one band, Nc >= 8 and tiles whose staged G blocks fit the
   // workgroup's LDS.
@@ -738,11 +865,14 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
       h->tile_order = atoi(to);
     if (const char* gr = getenv("G4ACC_GRAPH"))
       h->use_graph = atoi(gr) != 0;
-    h->use_tiled = g.nb == 1 && g.nc >= 8 && tiled_pairs_per_thread(g.nc) > 0 &&
+    // The tile kernels take a real sign and the spin-symmetric channels: the general model runs
+    // the per-entry kernel.
+    h->use_tiled = g.nb == 1 && g.nc >= 8 && tiled_pairs_per_thread(g.nc) > 0 && !h->general() &&
                    lds <= static_cast<size_t>(max_lds) && getenv("G4ACC_GENERIC_G4") == nullptr;
     const size_t R = static_cast<size_t>(g.nb) * g.nc;
     const size_t band_lds = cb * (6 * R * (R + 1) + 4 * static_cast<size_t>(g.nb) * g.nb * g.nc);
-    h->use_band = g.nb > 1 && band_lds <= static_cast<size_t>(max_lds) && getenv("G4ACC_GENERIC_G4") == nullptr;
+    h->use_band = g.nb > 1 && band_lds <= static_cast<size_t>(max_lds) && !h->general() &&
+                  getenv("G4ACC_GENERIC_G4") == nullptr;
     if (h->use_tiled &&
         (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_UV),
                        cb * W2 / 2 * 4 * g.nq * g.nv * g.nc * g.nw, "UV")))
@@ -752,8 +882,9 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
   for (auto& p : h->d_g4)
     if ((rc = h->alloc(reinterpret_cast<double2**>(&p), h->g4_entries, "G4")))
       return bail(rc);
-  h->stage_f_cap = W2 * (NM * NM + 2 * NM);  // M (or N), tau and exp(V) per slot
-  h->stage_i_cap = W2 * 2 * NM;
+  // Per slot: M (or N; complex M takes two doubles per entry), tau and exp(V); right and left labels.
+  h->stage_f_cap = W2 * ((h->complex_m ? 2 : 1) * NM * NM + 2 * NM);
+  h->stage_i_cap = W2 * (h->non_dd ? 4 : 2) * NM;
   if ((rc = h->alloc(&h->d_stage_f, h->stage_f_cap, "stage_f")) ||
       (rc = h->alloc(&h->d_stage_i, h->stage_i_cap, "stage_i")))
     return bail(rc);
@@ -769,9 +900,6 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
   // vertices by that label and the DFT columns follow it), so the right site DFT can run as two
   // L-point passes (G4ACC_DENSE_DFT=1 keeps the caller's labels and the dense DFT).
   const std::vector<int> tperm = diagonal_tile_order(g.nw, 2);
-  std::vector<int> xy, kxy_to_k;
-  if (getenv("G4ACC_DENSE_DFT") == nullptr)
-    h->grid_L = detect_grid(g.nc, gm->dft, xy, kxy_to_k);
   std::vector<double> dft_lab(gm->dft, gm->dft + 2 * static_cast<size_t>(nc2));
   if (h->grid_L) {
     for (int r = 0; r < g.nc; ++r)
@@ -813,7 +941,7 @@ int g4acc_create(const g4acc_geometry* gm, const g4acc_options* opt, g4acc_handl
     return bail(rc);
   // Default launch shapes for callers that do not tune (the C++ drop-in): the matrix-core forms
   // where they were measured faster, complex64 G on 16-site clusters with batches of 8 or more.
-  if (h->precision != G4ACC_FP64 && g.nb == 1 && g.nc == 16) {
+  if (h->precision != G4ACC_FP64 && g.nb == 1 && g.nc == 16 && !h->general()) {
     h->mtog_mfma = 1;
     int mask = 0;
     for (int c : h->channels)
@@ -877,8 +1005,7 @@ namespace {
 
 // accumulate_batch over walkers whose M is given as it is (exp_v == nullptr) or as N with the
 // vertices' exp(V) values (exp_v[2 i + s]).
-int accumulate_batch_impl(g4acc_handle* h, const g4acc_walker* walkers, const double* const* exp_v,
-                          int n_walkers, int memory) {
+int check_batch(g4acc_handle* h, int n_walkers, const void* walkers, int memory) {
   if (!h)
     return G4ACC_INVALID_ARG;
   if (!h->initialized)
@@ -886,27 +1013,38 @@ int accumulate_batch_impl(g4acc_handle* h, const g4acc_walker* walkers, const do
   if (n_walkers < 0 || (n_walkers > 0 && !walkers) ||
       (memory != G4ACC_HOST && memory != G4ACC_DEVICE_M && memory != G4ACC_DEVICE))
     return h->fail(G4ACC_INVALID_ARG, "invalid batch arguments");
+  return G4ACC_OK;
+}
+
+// Runs the non-empty walkers in chunks of max_walkers (empty configurations are no-ops,
+// tp_accumulator_cpu.hpp:238-239).
+int accumulate_walkers(g4acc_handle* h, const std::vector<WalkerIn>& all, int memory) {
   if (!h->g4_present)
     return G4ACC_OK;  // after sumTo the reference's G4 vector is empty: nothing is accumulated
   G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
-  std::vector<const g4acc_walker*> todo;
-  std::vector<const double* const*> todo_x;
-  todo.reserve(n_walkers);
-  for (int i = 0; i < n_walkers; ++i) {
-    G4_TRY(check_walker(h, walkers[i]));
-    if (walkers[i].n[0] + walkers[i].n[1] == 0)
-      continue;  // empty configuration: no-op (tp_accumulator_cpu.hpp:238-239)
-    todo.push_back(&walkers[i]);
-    if (exp_v)
-      todo_x.push_back(exp_v + 2 * i);
-  }
+  std::vector<WalkerIn> todo;
+  todo.reserve(all.size());
+  for (const WalkerIn& w : all)
+    if (w.n[0] + w.n[1] > 0)
+      todo.push_back(w);
   for (size_t i0 = 0; i0 < todo.size(); i0 += h->max_walkers) {
     const int nw_ = static_cast<int>(std::min<size_t>(h->max_walkers, todo.size() - i0));
-    G4_TRY(stage_walkers(h, todo.data() + i0, nw_, memory, h->stream,
-                         exp_v ? todo_x.data() + i0 : nullptr));
+    G4_TRY(stage_walkers(h, todo.data() + i0, nw_, memory, h->stream));
     G4_TRY(run_batch(h, nw_));
   }
   return G4ACC_OK;
+}
+
+int accumulate_batch_impl(g4acc_handle* h, const g4acc_walker* walkers, const double* const* exp_v,
+                          int n_walkers, int memory) {
+  G4_TRY(check_batch(h, n_walkers, walkers, memory));
+  std::vector<WalkerIn> all;
+  all.reserve(n_walkers);
+  for (int i = 0; i < n_walkers; ++i) {
+    all.push_back(walker_in(walkers[i], exp_v ? exp_v + 2 * i : nullptr));
+    G4_TRY(check_walker(h, all.back()));
+  }
+  return accumulate_walkers(h, all, memory);
 }
 
 }  // namespace
@@ -915,6 +1053,18 @@ extern "C" {
 
 int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_walkers, int memory) {
   return accumulate_batch_impl(h, walkers, nullptr, n_walkers, memory);
+}
+
+int g4acc_accumulate_general(g4acc_handle* h, const g4acc_walker_general* walkers, int n_walkers,
+                             int memory) {
+  G4_TRY(check_batch(h, n_walkers, walkers, memory));
+  std::vector<WalkerIn> all;
+  all.reserve(n_walkers);
+  for (int i = 0; i < n_walkers; ++i) {
+    all.push_back(walker_in(h, walkers[i]));
+    G4_TRY(check_walker(h, all.back(), true));
+  }
+  return accumulate_walkers(h, all, memory);
 }
 
 int g4acc_accumulate_batch_n(g4acc_handle* h, const g4acc_walker* walkers, const double* const* exp_v,
@@ -1146,19 +1296,21 @@ size_t g4acc_g_size(const g4acc_handle* h) {
   return h ? 2 * static_cast<size_t>(h->g.D) * h->g.D : 0;
 }
 
-int g4acc_compute_G(g4acc_handle* h, const g4acc_walker* walker, int memory, double* out,
-                    size_t n_complex) {
-  if (!h || !walker || !out)
-    return G4ACC_INVALID_ARG;
+}  // extern "C"
+
+namespace {
+
+// M -> G of one walker, exported in the reference layout to host memory.
+int compute_G_impl(g4acc_handle* h, const WalkerIn& w, bool general, int memory, double* out,
+                   size_t n_complex) {
   if (n_complex < g4acc_g_size(h))
     return h->fail(G4ACC_INVALID_ARG, "output buffer too small");
   if (memory != G4ACC_HOST && memory != G4ACC_DEVICE_M && memory != G4ACC_DEVICE)
     return h->fail(G4ACC_INVALID_ARG, "invalid memory kind");
   G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
-  G4_TRY(check_walker(h, *walker));
+  G4_TRY(check_walker(h, w, general));
   G4_TRY(quiesce(h));
-  const g4acc_walker* ws[1] = {walker};
-  G4_TRY(stage_walkers(h, ws, 1, memory, h->stream));
+  G4_TRY(stage_walkers(h, &w, 1, memory, h->stream));
   if (h->precision == G4ACC_FP64)
     run_mtog<double>(h, 2, h->stream);
   else
@@ -1175,6 +1327,24 @@ int g4acc_compute_G(g4acc_handle* h, const g4acc_walker* walker, int memory, dou
     rc = h->check(hipMemcpy(out, tmp, sizeof(double2) * n, hipMemcpyDeviceToHost), "copy G");
   (void)hipFree(tmp);
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int g4acc_compute_G(g4acc_handle* h, const g4acc_walker* walker, int memory, double* out,
+                    size_t n_complex) {
+  if (!h || !walker || !out)
+    return G4ACC_INVALID_ARG;
+  return compute_G_impl(h, walker_in(*walker), false, memory, out, n_complex);
+}
+
+int g4acc_compute_G_general(g4acc_handle* h, const g4acc_walker_general* walker, int memory, double* out,
+                            size_t n_complex) {
+  if (!h || !walker || !out)
+    return G4ACC_INVALID_ARG;
+  return compute_G_impl(h, walker_in(h, *walker), true, memory, out, n_complex);
 }
 
 }  // extern "C"
@@ -1224,9 +1394,9 @@ void launch_mtog_noalloc(g4acc_handle* h, const NoAllocLayout& l, int n_max, dou
   a.desc = reinterpret_cast<const SlotDesc*>(s + l.desc);
   a.n_max = n_max;
   a.chunk = 2;  // both spin sectors in one pass over the workspace
-  a.perm = perm;
-  a.off = off;
-  a.T = reinterpret_cast<Cplx<Real>*>(s + l.T);
+  a.perm = a.perm2 = perm;
+  a.off = a.off2 = off;
+  a.T = a.T2 = reinterpret_cast<Cplx<Real>*>(s + l.T);
   a.A = reinterpret_cast<Cplx<Real>*>(s + l.A);
   a.B = h->fused_mtog ? nullptr : reinterpret_cast<Cplx<Real>*>(s + l.B);
   a.E = h->fused_mtog ? nullptr : reinterpret_cast<Cplx<Real>*>(s + l.E);
@@ -1247,6 +1417,8 @@ int g4acc_compute_G_noalloc(g4acc_handle* h, const g4acc_walker* walker, double*
     return h->fail(G4ACC_INVALID_ARG, "output buffer too small");
   if (!h->d_scratch)
     return h->fail(G4ACC_DEVICE_ERROR, "the static workspace has no device address: " + h->scratch_err);
+  if (h->general())
+    return h->fail(G4ACC_INVALID_ARG, "compute_G_noalloc serves handles of the default model only");
   G4_TRY(h->check(hipSetDevice(h->device), "hipSetDevice"));
   G4_TRY(check_walker(h, *walker));
   const int n_max = std::max(1, std::max(walker->n[0], walker->n[1]));
@@ -1267,6 +1439,9 @@ int g4acc_compute_G_noalloc(g4acc_handle* h, const g4acc_walker* walker, double*
     d.ld = walker->ld[s];
     d.sign = walker->sign;
     d.expv = nullptr;
+    d.site_l = d.band_l = nullptr;
+    d.sign_im = 0;
+    d.m_cplx = 0;
     d.m = d.n ? walker->m[s] : nullptr;
     d.tau = d.n ? walker->tau[s] : nullptr;
     d.site = d.n ? walker->site[s] : nullptr;
@@ -1300,7 +1475,7 @@ int g4acc_accumulate_G(g4acc_handle* h, const double* G_ref, double sign) {
   int rc = h->check(hipMemcpyAsync(tmp, G_ref, sizeof(double2) * n, hipMemcpyHostToDevice, h->stream),
                     "copy G");
   if (rc == G4ACC_OK) {
-    h->h_desc[0] = SlotDesc{nullptr, nullptr, nullptr, nullptr, 1, 1, sign, nullptr};
+    h->h_desc[0] = SlotDesc{nullptr, nullptr, nullptr, nullptr, 1, 1, sign, nullptr, nullptr, nullptr, 0.0, 0};
     h->h_desc[1] = h->h_desc[0];
     rc = h->check(hipMemcpyAsync(h->d_desc, h->h_desc, 2 * sizeof(SlotDesc), hipMemcpyHostToDevice,
                                  h->stream), "copy desc");
@@ -1328,6 +1503,8 @@ int g4acc_set_tuning(g4acc_handle* h, const char* key, int value) {
     h->mtog_tw = value;
   else if (k == "mtog_mfma" && (value == 0 || value == 1))
     h->mtog_mfma = value;
+  else if (k == "mtog_large" && (value == 0 || (value == 1 && h->large_ok)))
+    h->mtog_large = value;
   else if (k == "g4_qg" && (value == 0 || value == 1 || value == 2 || value == 4))
     h->g4_qg = value;
   else if (k == "g4_tile_order" && value >= -1 && value <= 2)
@@ -1387,6 +1564,8 @@ int g4acc_sp_enable(g4acc_handle* h, int n_sp_freq, const int* r_sub, int accumu
     return G4ACC_INVALID_ARG;
   if (h->sp)
     return h->fail(G4ACC_INVALID_ARG, "the single-particle accumulator is already enabled");
+  if (h->general())
+    return h->fail(G4ACC_INVALID_ARG, "the single-particle accumulator serves handles of the default model only");
   if (n_sp_freq < 1 || !r_sub)
     return h->fail(G4ACC_INVALID_ARG, "n_sp_freq >= 1 and r_sub are required");
   const int nc = h->g.nc;

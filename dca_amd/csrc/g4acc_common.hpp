@@ -78,6 +78,13 @@ struct SlotDesc {
   // Non-null: m holds the walker's N matrix and M = diag(expv - 1) N is formed where M is read
   // (CtauxWalker::computeM, ctaux_walker.hpp:1889-1916); expv[i] = exp(V) of vertex i.
   const double* expv;
+  // General model: the vertices' left site and band (nullptr: the same as site / band, i.e.
+  // density-density vertices); site / band are their right labels, which order the rows of M, and
+  // the left ones order its columns (cached_ndft_base.hpp:97-103).
+  const int* site_l;
+  const int* band_l;
+  double sign_im;      // imaginary part of a complex sign (0 for real models)
+  int m_cplx;          // m holds interleaved complex entries (ld counts complex entries)
 };
 
 // Static geometry passed by value to kernels.

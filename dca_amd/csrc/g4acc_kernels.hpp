@@ -15,9 +15,17 @@ struct MtoGArgs {
   bool fused;             // use the register-fused single-band kernels
   int tw;                 // frequencies per thread in the fused right kernel (0: default)
   int mfma;               // fused right kernel on the matrix cores (complex64, Nc <= 16)
-  const int* perm;        // [n_slots][n_max]
+  // 6x6 and 8x8 clusters: S3 on the matrix cores into the E workspace, then the site transform
+  // and the G epilogue (mtog_large.hip), instead of the register-fused right kernel.
+  bool large = false;
+  const int* perm;        // [n_slots][n_max]  rows of M, sorted by the vertices' right labels
   const int* off;         // [n_slots][n_orb + 1]
   Cplx<Real>* T;          // [n_slots][n_max][nwE]
+  // Columns of M, sorted by the vertices' left labels, and their phases: the same arrays as
+  // perm / off / T unless the vertices are non-density-density (four-stage kernels only).
+  const int* perm2;
+  const int* off2;
+  Cplx<Real>* T2;
   Cplx<Real>* B;          // [chunk][n_max][D]
   Cplx<Real>* A;          // [chunk][n_max][D]
   Cplx<Real>* E;          // [chunk][D][D]
@@ -36,8 +44,13 @@ void launch_compute_m(int n, const double* exp_v, const double* N, int ldn, doub
                       hipStream_t st);
 
 // site_map (or nullptr): the internal label of each caller site (sites sorted by it).
+// left = true sorts by the vertices' left labels (where a descriptor has them).
 void launch_sort(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, const int* site_map,
-                 int* perm, int* off, int* err_flag, hipStream_t st);
+                 int* perm, int* off, int* err_flag, hipStream_t st, bool left = false);
+// Phases T[slot][w + nwE ii] = exp(i w_w tau_{perm ii}) of the vertices in the order of perm.
+template <typename Real>
+void launch_phases(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, const int* perm,
+                   Cplx<Real>* T, hipStream_t st);
 
 template <typename Real>
 void launch_mtog(const MtoGArgs<Real>& a, hipStream_t st);
@@ -45,6 +58,10 @@ void launch_mtog(const MtoGArgs<Real>& a, hipStream_t st);
 template <typename Real>
 bool launch_mtog_fused(const MtoGArgs<Real>& a, hipStream_t st);
 bool mtog_fused_covers(const Geo& g);
+// Right half of M -> G for single-band 6x6 (NC = 36) and 8x8 (NC = 64) clusters with site labels
+// x + L y, on slots [s0, s0 + ns) of a chunk (needs the E workspace).
+template <typename Real, int NC>
+void launch_mtog_large(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st);
 
 enum SumMode { SUM_FP64 = 0, SUM_FP32_KAHAN = 1 };
 
@@ -61,6 +78,9 @@ struct G4Args {
   const int* w_ex;             // device [nv]
   int sum_mode;
   void* g4;                    // double2[size] (FP64) or float4[size] (value, compensation)
+  // false: the updateG4 branch of models that are not spin symmetric (pp and magnetic, spin
+  // sector 0 only, tp_accumulator_cpu.hpp:668-757)
+  bool spin_symmetric = true;
 };
 
 template <typename Real>

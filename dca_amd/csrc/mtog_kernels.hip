@@ -34,7 +34,8 @@ namespace g4acc {
 __global__ void __launch_bounds__(256) sort_kernel(const SlotDesc* __restrict__ desc, Geo g,
                                                    int n_max, const int* __restrict__ site_map,
                                                    int* __restrict__ perm,
-                                                   int* __restrict__ off, int* __restrict__ err_flag) {
+                                                   int* __restrict__ off, int* __restrict__ err_flag,
+                                                   bool left) {
   extern __shared__ int lds[];
   int* orb = lds;               // n_max
   int* cnt = lds + n_max;       // n_orb + 1
@@ -46,8 +47,10 @@ __global__ void __launch_bounds__(256) sort_kernel(const SlotDesc* __restrict__ 
   for (int o = threadIdx.x; o <= g.n_orb; o += blockDim.x)
     cnt[o] = 0;
   __syncthreads();
+  const int* bands = left && d.band_l ? d.band_l : d.band;
+  const int* sites = left && d.site_l ? d.site_l : d.site;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int b = d.band[i], r = d.site[i];
+    const int b = bands[i], r = sites[i];
     int o = 0;
     if (b < 0 || b >= g.nb || r < 0 || r >= g.nc)
       atomicOr(err_flag, 1);
@@ -95,12 +98,15 @@ __global__ void __launch_bounds__(256) phase_kernel(const SlotDesc* __restrict__
   T[static_cast<size_t>(slot) * g.nwE * n_max + idx] = Cplx<Real>{static_cast<Real>(c), static_cast<Real>(s)};
 }
 
-// S1: B[slot][q + D jj] = sum_{ii in orbital o} T[w, ii] M[perm ii, perm jj], q = o + n_orb w.
+// S1: B[slot][q + D jj] = sum_{ii in orbital o} T[w, ii] M[perm ii, perm2 jj], q = o + n_orb w.
+// Rows of M in the order of perm (right labels), columns in the order of perm2 (left labels; the
+// same sort for density-density vertices). M real, or complex (d.m_cplx).
 template <typename Real>
 __global__ void __launch_bounds__(256) ndft_left_kernel(const SlotDesc* __restrict__ desc, Geo g,
                                                         int n_max, int slot0,
                                                         const int* __restrict__ perm,
                                                         const int* __restrict__ off,
+                                                        const int* __restrict__ perm2,
                                                         const Cplx<Real>* __restrict__ T,
                                                         Cplx<Real>* __restrict__ B) {
   using C = Cplx<Real>;
@@ -114,15 +120,25 @@ __global__ void __launch_bounds__(256) ndft_left_kernel(const SlotDesc* __restri
   const int* P = perm + static_cast<size_t>(slot) * n_max;
   const int* O = off + static_cast<size_t>(slot) * (g.n_orb + 1);
   const C* Ts = T + static_cast<size_t>(slot) * g.nwE * n_max;
-  const double* mcol = d.m + static_cast<size_t>(d.ld) * P[jj];
-  const double* xv = d.expv;  // N and exp(V): M_ij = (exp(V_i) - 1) N_ij, formed in FP64
+  const int j = perm2[static_cast<size_t>(slot) * n_max + jj];
   C acc{0, 0};
-  for (int ii = O[o]; ii < O[o + 1]; ++ii) {
-    const int i = P[ii];
-    const Real m = static_cast<Real>(xv ? (xv[i] - 1.0) * mcol[i] : mcol[i]);
-    const C t = Ts[w + static_cast<size_t>(g.nwE) * ii];
-    acc.x = fma(t.x, m, acc.x);
-    acc.y = fma(t.y, m, acc.y);
+  if (d.m_cplx) {
+    const double2* mcol = reinterpret_cast<const double2*>(d.m) + static_cast<size_t>(d.ld) * j;
+    for (int ii = O[o]; ii < O[o + 1]; ++ii) {
+      const double2 m = mcol[P[ii]];
+      cfma(acc, Ts[w + static_cast<size_t>(g.nwE) * ii], C{static_cast<Real>(m.x), static_cast<Real>(m.y)});
+    }
+  }
+  else {
+    const double* mcol = d.m + static_cast<size_t>(d.ld) * j;
+    const double* xv = d.expv;  // N and exp(V): M_ij = (exp(V_i) - 1) N_ij, formed in FP64
+    for (int ii = O[o]; ii < O[o + 1]; ++ii) {
+      const int i = P[ii];
+      const Real m = static_cast<Real>(xv ? (xv[i] - 1.0) * mcol[i] : mcol[i]);
+      const C t = Ts[w + static_cast<size_t>(g.nwE) * ii];
+      acc.x = fma(t.x, m, acc.x);
+      acc.y = fma(t.y, m, acc.y);
+    }
   }
   B[static_cast<size_t>(blockIdx.z) * g.D * n_max + q + static_cast<size_t>(g.D) * jj] = acc;
 }
@@ -149,7 +165,8 @@ __global__ void __launch_bounds__(256) dft_left_kernel(const SlotDesc* __restric
   A[base + p] = acc;
 }
 
-// S3: E[slot][p1 + D q2] = sum_{jj in orbital o2} A[p1 + D jj] conj(T[w2, jj]), q2 = o2 + n_orb w2.
+// S3: E[slot][p1 + D q2] = sum_{jj in orbital o2} A[p1 + D jj] conj(T[w2, jj]), q2 = o2 + n_orb w2,
+// over the column sort (off, T are the column order's arrays).
 template <typename Real>
 __global__ void __launch_bounds__(256) ndft_right_kernel(const SlotDesc* __restrict__ desc, Geo g,
                                                          int n_max, int slot0,
@@ -280,27 +297,37 @@ void launch_compute_m(int n, const double* exp_v, const double* N, int ldn, doub
 }
 
 void launch_sort(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, const int* site_map,
-                 int* perm, int* off, int* err_flag, hipStream_t st) {
+                 int* perm, int* off, int* err_flag, hipStream_t st, bool left) {
   const size_t lds = sizeof(int) * (n_max + g.n_orb + 1);
   hipLaunchKernelGGL(sort_kernel, dim3(n_slots), dim3(256), lds, st, desc, g, n_max, site_map, perm, off,
-                     err_flag);
+                     err_flag, left);
+}
+
+template <typename Real>
+void launch_phases(const SlotDesc* desc, const Geo& g, int n_slots, int n_max, const int* perm,
+                   Cplx<Real>* T, hipStream_t st) {
+  hipLaunchKernelGGL(phase_kernel<Real>, dim3((g.nwE * n_max + 255) / 256, n_slots), dim3(256), 0, st, desc, g,
+                     n_max, perm, T);
 }
 
 template <typename Real>
 void launch_mtog(const MtoGArgs<Real>& a, hipStream_t st) {
   const Geo& g = a.g;
-  hipLaunchKernelGGL(phase_kernel<Real>, dim3((g.nwE * a.n_max + 255) / 256, a.n_slots), dim3(256), 0, st,
-                     a.desc, g, a.n_max, a.perm, a.T);
-  if (a.fused && launch_mtog_fused<Real>(a, st))
+  launch_phases<Real>(a.desc, g, a.n_slots, a.n_max, a.perm, a.T, st);
+  const bool two_sorts = a.perm2 != a.perm;
+  if (two_sorts)
+    launch_phases<Real>(a.desc, g, a.n_slots, a.n_max, a.perm2, a.T2, st);
+  // The register-fused kernels read one sort and real M; the host never selects them otherwise.
+  if (a.fused && !two_sorts && launch_mtog_fused<Real>(a, st))
     return;
   for (int s0 = 0; s0 < a.n_slots; s0 += a.chunk) {
     const int ns = (a.n_slots - s0) < a.chunk ? (a.n_slots - s0) : a.chunk;
     hipLaunchKernelGGL(ndft_left_kernel<Real>, dim3((g.D + 255) / 256, a.n_max, ns), dim3(256), 0, st,
-                       a.desc, g, a.n_max, s0, a.perm, a.off, a.T, a.B);
+                       a.desc, g, a.n_max, s0, a.perm, a.off, a.perm2, a.T, a.B);
     hipLaunchKernelGGL(dft_left_kernel<Real>, dim3((g.D + 255) / 256, a.n_max, ns), dim3(256), 0, st,
                        a.desc, g, a.n_max, s0, a.dft, a.B, a.A);
     hipLaunchKernelGGL(ndft_right_kernel<Real>, dim3((g.D + 255) / 256, g.D, ns), dim3(256), 0, st,
-                       a.desc, g, a.n_max, s0, a.off, a.T, a.A, a.E);
+                       a.desc, g, a.n_max, s0, a.off2, a.T2, a.A, a.E);
     const int KW = g.nc * g.nwE;
     const dim3 grid((KW + 255) / 256, KW, ns);
     switch (g.nb) {
@@ -314,5 +341,7 @@ void launch_mtog(const MtoGArgs<Real>& a, hipStream_t st) {
 
 template void launch_mtog<double>(const MtoGArgs<double>&, hipStream_t);
 template void launch_mtog<float>(const MtoGArgs<float>&, hipStream_t);
+template void launch_phases<double>(const SlotDesc*, const Geo&, int, int, const int*, Cplx<double>*, hipStream_t);
+template void launch_phases<float>(const SlotDesc*, const Geo&, int, int, const int*, Cplx<float>*, hipStream_t);
 
 }  // namespace g4acc

@@ -1,0 +1,205 @@
+// Right half of M -> G for the large single-band square clusters (6x6 and 8x8), in two kernels:
+//
+//   mtog_s3_mfma  S3 on the matrix cores: for every site r of the column sort
+//                     E[w2][r][p1] = sum_{jj in site r} conj(T[w2, jj]) A[p1, jj]
+//                 written to a D x D workspace (per slot of the chunk);
+//   mtog_site_G   the site transform over r and the G epilogue (FP64)
+//                     G[p1, (k2, w2)] = -G0(p1) / Nc sum_r exp(-i k2.r) E[w2][r][p1] G0(k2, w2)
+//                                       + beta G0(p1) delta_{p1 p2}
+//                 as two L-point passes through LDS (sites labelled x + L y).
+//
+// Keeping E in registers, as the fused right kernels do, needs Nc complex sums per (p1, w2) pair
+// and thread; at Nc = 64 that leaves one or three waves per SIMD and no room for matrix-core tiles.
+// Written out, E costs one D x D write and read per slot, and both kernels run at full occupancy:
+// S3 as 16 x 16 x 4 matrix-core steps of four vertices, the site transform with eight (or six)
+// values per thread.
+#include "g4acc_common.hpp"
+#include "g4acc_kernels.hpp"
+#include "site_fft.hpp"
+
+namespace g4acc {
+
+namespace {
+
+template <typename Real>
+struct Mfma16;
+// Output row of register i of a lane in lane group jl = lane / 16 (the column is lane % 16). The
+// two instructions differ (measured on gfx950): FP32 holds four consecutive rows per lane, FP64
+// rows strided by four.
+template <>
+struct Mfma16<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int jl, int i) { return 4 * jl + i; }
+};
+template <>
+struct Mfma16<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int jl, int i) { return jl + 4 * i; }
+};
+
+}  // namespace
+
+// One workgroup owns 64 rows p1 (16 per wave) and 64 frequencies w2 (four 16 x 16 tiles per wave)
+// and walks the sites. A step takes four vertices of one site: lane l supplies conj-T[w][jj0 + l/16]
+// for w = w0 + 16 t + l%16 and A[p0 + l%16][jj0 + l/16], and after the site holds
+// E[w0 + 16 t + row(l/16, i)][p0 + l%16], i = 0..3, which it stores as 16-entry runs of p1.
+// Loads past the site's vertices or the tile's edges read clamped, in-bounds addresses and are
+// zeroed before use.
+template <typename Real, int NC>
+__global__ void __launch_bounds__(256) mtog_s3_mfma(Geo g, int n_max, int slot0,
+                                                    const int* __restrict__ off,
+                                                    const Cplx<Real>* __restrict__ T,
+                                                    const Cplx<Real>* __restrict__ A,
+                                                    Cplx<Real>* __restrict__ E) {
+  using C = Cplx<Real>;
+  using V = typename Mfma16<Real>::acc_t;
+  const int slot = slot0 + blockIdx.z;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int D = g.D, nwE = g.nwE;
+  const int* O = off + static_cast<size_t>(slot) * (NC + 1);
+  const C* Ts = T + static_cast<size_t>(slot) * nwE * n_max;
+  const C* Ab = A + static_cast<size_t>(blockIdx.z) * D * n_max;
+  C* Eb = E + static_cast<size_t>(blockIdx.z) * D * D;
+  const int jl = lane >> 4;
+  const int pl = blockIdx.x * 64 + 16 * wave + (lane & 15);
+  const bool p_ok = pl < D;
+  const int pc = p_ok ? pl : D - 1;
+  const int w0 = blockIdx.y * 64;
+  int wc[4];
+  bool w_ok[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int w = w0 + 16 * t + (lane & 15);
+    w_ok[t] = w < nwE;
+    wc[t] = w_ok[t] ? w : nwE - 1;
+  }
+  for (int r = 0; r < NC; ++r) {
+    const int b = O[r], e = O[r + 1];
+    V cr[4], ci[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      cr[t] = ci[t] = V{0, 0, 0, 0};
+    for (int j0 = b; j0 < e; j0 += 4) {
+      const int jj = j0 + jl;
+      const bool j_ok = jj < e;
+      const int jc = j_ok ? jj : b;
+      C a = Ab[pc + static_cast<size_t>(D) * jc];
+      if (!(j_ok && p_ok))
+        a = C{0, 0};
+      C tt[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        tt[t] = Ts[wc[t] + static_cast<size_t>(nwE) * jc];
+        if (!(j_ok && w_ok[t]))
+          tt[t] = C{0, 0};
+      }
+      // E += conj(T) A = (Tx Ax + Ty Ay) + i (Tx Ay - Ty Ax)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        cr[t] = Mfma16<Real>::mma(tt[t].x, a.x, cr[t]);
+        ci[t] = Mfma16<Real>::mma(tt[t].x, a.y, ci[t]);
+        cr[t] = Mfma16<Real>::mma(tt[t].y, a.y, cr[t]);
+        ci[t] = Mfma16<Real>::mma(-tt[t].y, a.x, ci[t]);
+      }
+    }
+    if (p_ok) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int w = w0 + 16 * t + Mfma16<Real>::row(jl, i);
+          if (w < nwE)
+            Eb[pl + static_cast<size_t>(D) * (r + NC * w)] = C{cr[t][i], ci[t][i]};
+        }
+    }
+  }
+}
+
+// One workgroup owns 64 rows p1 (one per lane) and one frequency w2. Pass 1: wave q transforms
+// the rows y = q, q + 4, ... over x into LDS; pass 2: it transforms the columns kx = q, q + 4, ...
+// over y and writes G for the L momenta (kx, ky) of each.
+template <typename Real, int L>
+__global__ void __launch_bounds__(256) mtog_site_G(Geo g, int slot0, const Cplx<Real>* __restrict__ E,
+                                                   const double2* __restrict__ bphase,
+                                                   const double2* __restrict__ G0,
+                                                   const int* __restrict__ kmap,
+                                                   Cplx<Real>* __restrict__ G) {
+  using C = Cplx<Real>;
+  constexpr int NC = L * L;
+  __shared__ C F[NC * 64];  // [kx + L y][lane]
+  __shared__ int Kl[NC];
+  const int slot = slot0 + blockIdx.z;
+  const int s = slot & 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int D = g.D;
+  const int p1 = blockIdx.x * 64 + lane;
+  const bool p_ok = p1 < D;
+  const int pc = p_ok ? p1 : D - 1;
+  const int w2 = blockIdx.y;
+  const C* Eb = E + static_cast<size_t>(blockIdx.z) * D * D + static_cast<size_t>(D) * NC * w2 + pc;
+  if (threadIdx.x < NC)
+    Kl[threadIdx.x] = kmap[threadIdx.x];
+  for (int y = wave; y < L; y += 4) {
+    C v[L];
+#pragma unroll
+    for (int x = 0; x < L; ++x)
+      v[x] = Eb[static_cast<size_t>(D) * (x + L * y)];
+    site_transform_1d<L, -1, Real>(v);
+#pragma unroll
+    for (int kx = 0; kx < L; ++kx)
+      F[(kx + L * y) * 64 + lane] = v[kx];
+  }
+  __syncthreads();
+  if (!p_ok)
+    return;
+  const int k1 = p1 % NC, w1 = p1 / NC;
+  const double2 ga = G0[s + 2 * (k1 + static_cast<size_t>(NC) * w1)];
+  const double inv_nc = 1.0 / NC;
+  const double2 gneg{-ga.x * inv_nc, -ga.y * inv_nc};  // -G0(p1) / Nc
+  C* Gs = G + static_cast<size_t>(slot) * D * D + p1;
+  for (int kx = wave; kx < L; kx += 4) {
+    C v[L];
+#pragma unroll
+    for (int y = 0; y < L; ++y)
+      v[y] = F[(kx + L * y) * 64 + lane];
+    site_transform_1d<L, -1, Real>(v);
+#pragma unroll
+    for (int ky = 0; ky < L; ++ky) {
+      const int k2 = Kl[kx + L * ky];
+      double2 m = to_d(v[ky]);
+      if (bphase)
+        m = cmulc(cmul(m, bphase[k1]), bphase[k2]);
+      double2 val = cmul(cmul(gneg, m), G0[s + 2 * (k2 + static_cast<size_t>(NC) * w2)]);
+      const int p2 = k2 + NC * w2;
+      if (p1 == p2) {
+        val.x = fma(ga.x, g.beta, val.x);
+        val.y = fma(ga.y, g.beta, val.y);
+      }
+      Gs[static_cast<size_t>(D) * p2] = from_d<C>(val);
+    }
+  }
+}
+
+template <typename Real, int NC>
+void launch_mtog_large(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st) {
+  constexpr int L = NC == 64 ? 8 : 6;
+  static_assert(L * L == NC, "6x6 and 8x8 clusters");
+  const Geo& g = a.g;
+  const dim3 g3((g.D + 63) / 64, (g.nwE + 63) / 64, ns);
+  hipLaunchKernelGGL((mtog_s3_mfma<Real, NC>), g3, dim3(256), 0, st, g, a.n_max, s0, a.off, a.T, a.A, a.E);
+  const dim3 gs((g.D + 63) / 64, g.nwE, ns);
+  hipLaunchKernelGGL((mtog_site_G<Real, L>), gs, dim3(256), 0, st, g, s0, a.E, a.bphase, a.G0, a.kmap, a.G);
+}
+
+template void launch_mtog_large<float, 36>(const MtoGArgs<float>&, int, int, hipStream_t);
+template void launch_mtog_large<float, 64>(const MtoGArgs<float>&, int, int, hipStream_t);
+template void launch_mtog_large<double, 36>(const MtoGArgs<double>&, int, int, hipStream_t);
+template void launch_mtog_large<double, 64>(const MtoGArgs<double>&, int, int, hipStream_t);
+
+}  // namespace g4acc

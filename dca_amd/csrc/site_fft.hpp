@@ -140,6 +140,30 @@ __host__ __device__ __forceinline__ void site_dft_2d(Cplx<Real> (&v)[L * L]) {
   }
 }
 
+// One L-point transform in place: x[k] <- sum_n x[n] exp(SIGN 2 pi i k n / L), radix 2 for
+// L = 2, 4, 8 and the direct sum of rotations for L = 6.
+template <int L, int SIGN, typename Real>
+__host__ __device__ __forceinline__ void site_transform_1d(Cplx<Real> (&x)[L]) {
+  if constexpr (L == 2 || L == 4 || L == 8) {
+    fft_pow2<L, SIGN, Real>(x, 0, 1);
+  }
+  else {
+    using C = Cplx<Real>;
+    C y[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      C acc = x[0];
+#pragma unroll
+      for (int n = 1; n < L; ++n)
+        acc = sf_add(acc, rot<L, Real>(x[n], -SIGN * k * n));
+      y[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < L; ++k)
+      x[k] = y[k];
+  }
+}
+
 // Radix-2 where it applies, the two-pass sum otherwise.
 template <int L, int SIGN, typename Real>
 __host__ __device__ __forceinline__ void site_transform_2d(Cplx<Real> (&v)[L * L]) {

@@ -36,6 +36,7 @@ EXPORTED = [
     "g4acc_accumulate_single",
     "g4acc_flush", "g4acc_sync_copy", "g4acc_accumulate_batch_n", "g4acc_compute_m",
     "g4acc_sp_enable", "g4acc_sp_get_m_r_w", "g4acc_sp_size", "g4acc_compute_G_noalloc",
+    "g4acc_create_model", "g4acc_accumulate_general", "g4acc_compute_G_general",
 ]
 
 
@@ -65,6 +66,18 @@ class Walker_c(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int * 2), ("m", ctypes.c_void_p * 2), ("ld", ctypes.c_int * 2),
                 ("tau", ctypes.c_void_p * 2), ("site", ctypes.c_void_p * 2),
                 ("band", ctypes.c_void_p * 2), ("sign", ctypes.c_double)]
+
+
+class Model_c(ctypes.Structure):
+    _fields_ = [("spin_symmetric", ctypes.c_int), ("complex_m", ctypes.c_int),
+                ("non_density_density", ctypes.c_int)]
+
+
+class WalkerGeneral_c(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 2), ("m", ctypes.c_void_p * 2), ("ld", ctypes.c_int * 2),
+                ("tau", ctypes.c_void_p * 2), ("site", ctypes.c_void_p * 2),
+                ("band", ctypes.c_void_p * 2), ("site_l", ctypes.c_void_p * 2),
+                ("band_l", ctypes.c_void_p * 2), ("sign", ctypes.c_double * 2)]
 
 
 _lib = None
@@ -102,6 +115,10 @@ def load_library(path: str = LIB_PATH):
     vp, sz, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
     sig = {
         "g4acc_create": (i32, [ctypes.POINTER(Geometry_c), ctypes.POINTER(Options_c), ctypes.POINTER(vp)]),
+        "g4acc_create_model": (i32, [ctypes.POINTER(Geometry_c), ctypes.POINTER(Options_c),
+                                     ctypes.POINTER(Model_c), ctypes.POINTER(vp)]),
+        "g4acc_accumulate_general": (i32, [vp, ctypes.POINTER(WalkerGeneral_c), i32, i32]),
+        "g4acc_compute_G_general": (i32, [vp, ctypes.POINTER(WalkerGeneral_c), i32, vp, sz]),
         "g4acc_destroy": (i32, [vp]),
         "g4acc_reset": (i32, [vp]),
         "g4acc_accumulate": (i32, [vp, ctypes.POINTER(Walker_c), i32]),
@@ -179,6 +196,39 @@ def walker_struct(meas, keep: _Keep) -> Walker_c:
     return w
 
 
+def is_general(meas) -> bool:
+    """A measurement of the general model: complex M or sign, or vertices with left labels."""
+    return (getattr(meas, "site_l", None) is not None or any(np.iscomplexobj(x) for x in meas.M)
+            or isinstance(meas.sign, complex))
+
+
+def general_walker_struct(meas, keep: _Keep, complex_m: bool) -> WalkerGeneral_c:
+    """Host-memory walker of the general model (complex M as interleaved doubles when complex_m)."""
+    w = WalkerGeneral_c()
+    nd = getattr(meas, "site_l", None) is not None
+    for s in range(2):
+        n = int(meas.n[s])
+        w.n[s] = n
+        w.ld[s] = max(n, 1)
+        if n:
+            if not complex_m and np.iscomplexobj(meas.M[s]):
+                raise LogicError("complex M needs a handle with complex_m")
+            m = keep.add(np.asfortranarray(meas.M[s], np.complex128 if complex_m else np.float64))
+            w.ld[s] = m.shape[0]
+            t = keep.add(np.ascontiguousarray(meas.tau[s], np.float64))
+            r = keep.add(np.ascontiguousarray(meas.site[s], np.int32))
+            b = keep.add(np.ascontiguousarray(meas.band[s], np.int32))
+            w.m[s], w.tau[s], w.site[s], w.band[s] = (m.ctypes.data, t.ctypes.data, r.ctypes.data,
+                                                      b.ctypes.data)
+            if nd:
+                rl = keep.add(np.ascontiguousarray(meas.site_l[s], np.int32))
+                bl = keep.add(np.ascontiguousarray(meas.band_l[s], np.int32))
+                w.site_l[s], w.band_l[s] = rl.ctypes.data, bl.ctypes.data
+    sign = complex(meas.sign)
+    w.sign[0], w.sign[1] = sign.real, sign.imag
+    return w
+
+
 def device_walker_struct(n: Sequence[int], m_ptr, ld, tau_ptr, site_ptr, band_ptr, sign) -> Walker_c:
     """Device-memory walker from raw device pointers (per spin)."""
     w = Walker_c()
@@ -197,7 +247,8 @@ class TpAccumulator:
     """g4acc-backed TpAccumulator (one HIP device, one stream)."""
 
     def __init__(self, geom: geo.Geometry, precision: str = "fp64", sum_mode: str = "fp64",
-                 max_walkers: int = 1, max_vertices: int = 64, device: int = 0, tuned: bool = True):
+                 max_walkers: int = 1, max_vertices: int = 64, device: int = 0, tuned: bool = True,
+                 spin_symmetric: bool = True, complex_m: bool = False, non_density_density: bool = False):
         self.lib = load_library()
         self.geom = geom
         self._keep = _Keep()
@@ -228,13 +279,16 @@ class TpAccumulator:
         self.precision, self.sum_mode = precision, sum_mode
         self.max_walkers, self.max_vertices = int(max_walkers), int(max_vertices)
         h = ctypes.c_void_p()
-        rc = self.lib.g4acc_create(ctypes.byref(gc), ctypes.byref(opt), ctypes.byref(h))
+        model = Model_c(int(spin_symmetric), int(complex_m), int(non_density_density))
+        self.general_model = not spin_symmetric or complex_m or non_density_density
+        self.complex_m = complex_m
+        rc = self.lib.g4acc_create_model(ctypes.byref(gc), ctypes.byref(opt), ctypes.byref(model), ctypes.byref(h))
         if rc != OK:
             _raise(rc, self.lib.g4acc_last_error(None).decode())
         self._h = h
         self._inflight = []
         self.tuning = None
-        if tuned:
+        if tuned and not self.general_model:
             # Launch shapes measured by dca_amd/autotune.py for this geometry and batch size.
             from . import autotune
             self.tuning = autotune.lookup(geom, precision, self.max_walkers)
@@ -262,6 +316,8 @@ class TpAccumulator:
 
     def accumulate(self, meas) -> None:
         """accumulate(M_pair, configs, sign) for one measurement in host memory."""
+        if self.general_model or is_general(meas):
+            return self.accumulate_batch([meas])
         keep = _Keep()
         w = walker_struct(meas, keep)
         self._check(self.lib.g4acc_accumulate(self._h, ctypes.byref(w), HOST))
@@ -269,8 +325,13 @@ class TpAccumulator:
 
     def accumulate_batch(self, measurements) -> None:
         keep = _Keep()
-        arr = (Walker_c * len(measurements))(*[walker_struct(m, keep) for m in measurements])
-        self._check(self.lib.g4acc_accumulate_batch(self._h, arr, len(measurements), HOST))
+        if self.general_model or any(is_general(m) for m in measurements):
+            arr = (WalkerGeneral_c * len(measurements))(
+                *[general_walker_struct(m, keep, self.complex_m) for m in measurements])
+            self._check(self.lib.g4acc_accumulate_general(self._h, arr, len(measurements), HOST))
+        else:
+            arr = (Walker_c * len(measurements))(*[walker_struct(m, keep) for m in measurements])
+            self._check(self.lib.g4acc_accumulate_batch(self._h, arr, len(measurements), HOST))
         self._inflight.append(keep)
 
     def accumulate_batch_n(self, measurements, exp_v) -> None:
@@ -378,10 +439,14 @@ class TpAccumulator:
     def compute_G(self, meas) -> np.ndarray:
         """G of one measurement in the reference layout (b1, b2, s, k1, k2, w1E, w2E)."""
         keep = _Keep()
-        w = walker_struct(meas, keep)
         n = self.lib.g4acc_g_size(self._h)
         out = np.empty(n, np.complex128)
-        self._check(self.lib.g4acc_compute_G(self._h, ctypes.byref(w), HOST, out.ctypes.data, n))
+        if self.general_model or is_general(meas):
+            w = general_walker_struct(meas, keep, self.complex_m)
+            self._check(self.lib.g4acc_compute_G_general(self._h, ctypes.byref(w), HOST, out.ctypes.data, n))
+        else:
+            w = walker_struct(meas, keep)
+            self._check(self.lib.g4acc_compute_G(self._h, ctypes.byref(w), HOST, out.ctypes.data, n))
         g = self.geom
         return out.reshape((g.nb, g.nb, 2, g.nc, g.nc, g.n_ext, g.n_ext), order="F")
 

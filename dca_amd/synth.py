@@ -72,11 +72,14 @@ class Stream:
 @dataclasses.dataclass
 class Measurement:
     n: list[int]
-    M: list[np.ndarray]      # Fortran-ordered (N, N) float64
+    M: list[np.ndarray]      # Fortran-ordered (N, N) float64 (complex128 for complex models)
     tau: list[np.ndarray]    # float64
-    site: list[np.ndarray]   # int32
-    band: list[np.ndarray]   # int32
-    sign: float
+    site: list[np.ndarray]   # int32, the vertices' right sites (both sides for density-density)
+    band: list[np.ndarray]   # int32, the vertices' right bands
+    sign: float              # +-1, or a complex phase for complex models
+    # Non-density-density vertices: their left sites and bands (None: the same as the right ones).
+    site_l: list[np.ndarray] | None = None
+    band_l: list[np.ndarray] | None = None
 
 
 def draw_measurement(rs: Stream, nb: int, nc: int, beta: float, draw_n: Callable[[Stream], int],
@@ -210,3 +213,29 @@ def walker_n_pair(meas: Measurement, rs: Stream, gamma: float = 0.6):
         xvs.append(xv)
     return Measurement(n=list(meas.n), M=Ns, tau=meas.tau, site=meas.site, band=meas.band,
                        sign=meas.sign), xvs
+
+
+def general_measurement(rs: Stream, nb: int, nc: int, beta: float, n: tuple[int, int],
+                        complex_m: bool = True, non_density: bool = True,
+                        phase: float | None = 0.7) -> Measurement:
+    """A measurement of the general model: M with complex N(0, 1/N) entries (real and imaginary
+    parts each N(0, 1/2N)), vertices whose left (site, band) are drawn apart from their right ones
+    (non-density-density vertices, as CT-INT's SectorEntry holds them), and the sign a complex
+    phase exp(i phase) (SignType of a complex model; phase None keeps +1)."""
+    Ms, taus, sites, bands, sites_l, bands_l = [], [], [], [], [], []
+    for s in range(2):
+        N = int(n[s])
+        u = rs.uniform(5 * N)
+        taus.append(u[:N] * beta)
+        sites.append(np.minimum((u[N:2 * N] * nc).astype(np.int32), nc - 1))
+        bands.append(np.minimum((u[2 * N:3 * N] * nb).astype(np.int32), nb - 1))
+        if non_density:
+            sites_l.append(np.minimum((u[3 * N:4 * N] * nc).astype(np.int32), nc - 1))
+            bands_l.append(np.minimum((u[4 * N:] * nb).astype(np.int32), nb - 1))
+        z = rs.normal(N * N) * math.sqrt(1.0 / N) if N else np.zeros(0)
+        if complex_m and N:
+            z = (z + 1j * rs.normal(N * N) * math.sqrt(1.0 / N)) * math.sqrt(0.5)
+        Ms.append(np.asfortranarray(z.reshape(N, N, order="F")))
+    sign = 1.0 if phase is None else complex(math.cos(phase), math.sin(phase))
+    return Measurement(n=[int(x) for x in n], M=Ms, tau=taus, site=sites, band=bands, sign=sign,
+                       site_l=sites_l if non_density else None, band_l=bands_l if non_density else None)

@@ -104,10 +104,45 @@ typedef struct g4acc_walker {
   double sign;              /* SignType: +1 / -1 */
 } g4acc_walker;
 
+/* Model properties the reference fixes at compile time from Parameters. g4acc_create is
+ * g4acc_create_model with {1, 0, 0}. */
+typedef struct g4acc_model {
+  int spin_symmetric;       /* lattice_type::spin_symmetric (tp_accumulator_base.hpp:167). 0: the
+                               updateG4 branch of models whose spin is part of the band index
+                               (tp_accumulator_cpu.hpp:668-757): G4 from spin sector 0 with the
+                               whole sign; only the pp and magnetic channels exist there */
+  int complex_m;            /* M complex (complex models; Matrix<std::complex<double>>) */
+  int non_density_density;  /* vertices whose left and right (site, band) differ
+                               (HasInitializeNonDensityInteractionMethod, tp_accumulator_base.hpp:169-170):
+                               rows of M follow the right labels, columns the left ones
+                               (cached_ndft_base.hpp:72-78, 97-103) */
+} g4acc_model;
+
+/* One measurement of the general model (accumulate(M_pair, configs, sign) with any SpScalar,
+ * tp_accumulator_cpu.hpp:231-252). */
+typedef struct g4acc_walker_general {
+  int n[2];                 /* vertices per spin sector */
+  const double* m[2];       /* M_pair[s], column-major; interleaved (re, im) when complex_m */
+  int ld[2];                /* leading dimensions in elements (complex entries when complex_m) */
+  const double* tau[2];     /* vertex times in [0, beta) */
+  const int* site[2];       /* Vertex::get_right_site() */
+  const int* band[2];       /* Vertex::get_right_band() */
+  const int* site_l[2];     /* Vertex::get_left_site() (read when non_density_density; NULL = site) */
+  const int* band_l[2];     /* Vertex::get_left_band() (read when non_density_density; NULL = band) */
+  double sign[2];           /* SignType (type_help.hpp:299-312): (re, im); a complex phase for
+                               complex models, (+-1, 0) otherwise */
+} g4acc_walker_general;
+
 typedef struct g4acc_handle g4acc_handle;
 
 /* Constructor (tp_accumulator_base.hpp:184-208, tp_accumulator_gpu_base.hpp:132-152). */
 int g4acc_create(const g4acc_geometry* geometry, const g4acc_options* options, g4acc_handle** out);
+/* Constructor for a model with the properties in *model. A handle whose model is not
+ * {1, 0, 0} runs the four-stage M -> G and the per-entry G4 kernels. With spin_symmetric = 0, a
+ * channel other than pp and magnetic is refused (G4ACC_UNSUPPORTED_CHANNEL, as the reference
+ * throws std::logic_error, tp_accumulator_cpu.hpp:752-753). */
+int g4acc_create_model(const g4acc_geometry* geometry, const g4acc_options* options,
+                       const g4acc_model* model, g4acc_handle** out);
 /* Destructor. */
 int g4acc_destroy(g4acc_handle* h);
 /* resetAccumulation (tp_accumulator_cpu.hpp:224-229, tp_accumulator_gpu.hpp:271-296). */
@@ -119,6 +154,11 @@ int g4acc_accumulate(g4acc_handle* h, const g4acc_walker* walker, int memory);
 /* Many walkers' measurements in one batched launch sequence; equivalent to calling
  * g4acc_accumulate for each walker. n_walkers may exceed max_walkers (processed in chunks). */
 int g4acc_accumulate_batch(g4acc_handle* h, const g4acc_walker* walkers, int n_walkers, int memory);
+/* g4acc_accumulate_batch for measurements of the general model (complex M, a complex sign, left
+ * and right vertex labels), on a handle from g4acc_create_model. Complex M needs complex_m; left
+ * labels are read only with non_density_density. Any finite sign is accepted. */
+int g4acc_accumulate_general(g4acc_handle* h, const g4acc_walker_general* walkers, int n_walkers,
+                             int memory);
 /* g4acc_accumulate_batch for walkers that hand over N and the vertices' exp(V) instead of M, as a
  * CT-AUX walker holds them (CtauxWalker::computeM, ctaux_walker.hpp:1889-1916): walkers[i].m[s]
  * points to N_s (leading dimension ld[s]) and exp_v[2 i + s] to the n[s] values exp(V_j) of its
@@ -176,6 +216,10 @@ const char* g4acc_last_error(const g4acc_handle* h);
  * M -> G step of one walker and copies G in the reference layout (b1,b2,s,k1,k2,w1E,w2E). */
 int g4acc_compute_G(g4acc_handle* h, const g4acc_walker* walker, int memory, double* out,
                     size_t n_complex);
+/* g4acc_compute_G for a measurement of the general model (get_G_Debug of a complex or
+ * non-density-density model). */
+int g4acc_compute_G_general(g4acc_handle* h, const g4acc_walker_general* walker, int memory, double* out,
+                            size_t n_complex);
 /* Complex entries of G in the reference layout. */
 size_t g4acc_g_size(const g4acc_handle* h);
 /* No counterpart in the reference. For callers that cannot allocate device memory (code under

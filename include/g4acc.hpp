@@ -16,7 +16,10 @@
 // density-density interactions (tp/ndft/cached_ndft_base.hpp:97-103). A vertex whose left and
 // right orbitals differ (a non-density-density interaction, which the reference handles with a
 // second sorted side, cached_ndft_base.hpp:72-78) raises std::logic_error instead of giving a
-// wrong G4.
+// wrong G4, unless the accumulator was built for such a model (g4acc_model::non_density_density),
+// which sorts the columns of M by the left labels. Complex M (Matrix<std::complex<double>>) and a
+// complex sign need g4acc_model::complex_m; a model that is not spin symmetric sets
+// g4acc_model::spin_symmetric = 0 (tp_accumulator_cpu.hpp:668-757).
 #ifndef G4ACC_HPP
 #define G4ACC_HPP
 
@@ -27,6 +30,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -69,6 +73,33 @@ void extract_configuration(const Configuration& conf, std::vector<double>& tau, 
     site[i] = v.get_right_site();
   }
 }
+
+// Both label sets of a configuration (non-density-density vertices): right labels order the rows
+// of M, left labels its columns (cached_ndft_base.hpp:97-103).
+template <class Configuration>
+void extract_configuration_general(const Configuration& conf, std::vector<double>& tau,
+                                   std::vector<int>& site, std::vector<int>& band,
+                                   std::vector<int>& site_l, std::vector<int>& band_l) {
+  const int n = static_cast<int>(conf.size());
+  tau.resize(n);
+  site.resize(n);
+  band.resize(n);
+  site_l.resize(n);
+  band_l.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const auto& v = conf[i];
+    tau[i] = v.get_tau();
+    band[i] = v.get_right_band();
+    site[i] = v.get_right_site();
+    band_l[i] = v.get_left_band();
+    site_l[i] = v.get_left_site();
+  }
+}
+
+template <class T>
+struct is_complex : std::false_type {};
+template <class T>
+struct is_complex<std::complex<T>> : std::true_type {};
 
 // G0 over the EXTENDED vertex frequencies, as TpAccumulatorBase::initializeG0 slices the
 // cluster-excluded single-particle G0 (tp_accumulator_base.hpp:210-221), in the (b1, b2, s, k, wE)
@@ -135,10 +166,14 @@ public:
 
   // queued = true: accumulate() hands each measurement to the handle's queue, which runs as one
   // batch of options.max_walkers measurements (g4acc_accumulate_queued).
-  TpAccumulator(const Geometry& geometry, const g4acc_options& options, bool queued = false)
-      : geometry_(geometry), queued_(queued) {
+  // model: the reference's compile-time model properties (spin symmetry, complex M, non-density-
+  // density vertices); the default is a real, spin-symmetric, density-density model. The queue
+  // serves the default model only.
+  TpAccumulator(const Geometry& geometry, const g4acc_options& options, bool queued = false,
+                g4acc_model model = g4acc_model{1, 0, 0})
+      : geometry_(geometry), queued_(queued), model_(model) {
     const g4acc_geometry g = geometry_.view();
-    throw_on_error(g4acc_create(&g, &options, &handle_), g4acc_last_error(nullptr));
+    throw_on_error(g4acc_create_model(&g, &options, &model_, &handle_), g4acc_last_error(nullptr));
   }
   ~TpAccumulator() {
     if (handle_)
@@ -149,6 +184,7 @@ public:
   TpAccumulator(TpAccumulator&& other) noexcept
       : geometry_(std::move(other.geometry_)),
         queued_(other.queued_),
+        model_(other.model_),
         handle_(std::exchange(other.handle_, nullptr)) {}
 
   // Zeroes G4 (tp_accumulator_cpu.hpp:224-229). The loop index is accepted for signature parity.
@@ -160,6 +196,10 @@ public:
   template <class MatrixPair, class Configuration, class Sign>
   double accumulate(const MatrixPair& M_pair, const std::array<Configuration, 2>& configs,
                     const Sign sign, int memory = G4ACC_HOST) {
+    using Scalar = std::remove_cv_t<std::remove_pointer_t<decltype(M_pair[0].ptr())>>;
+    if (model_.non_density_density || model_.complex_m || !model_.spin_symmetric ||
+        is_complex<Scalar>::value || is_complex<Sign>::value)
+      return accumulate_general(M_pair, configs, sign, memory);
     g4acc_walker w{};
     for (int s = 0; s < 2; ++s) {
       const auto& conf = configs[s];
@@ -175,6 +215,52 @@ public:
     w.sign = static_cast<double>(sign);
     const int kind = memory == G4ACC_HOST ? G4ACC_HOST : G4ACC_DEVICE_M;
     check(queued_ ? g4acc_accumulate_queued(handle_, &w, kind) : g4acc_accumulate(handle_, &w, kind));
+    return flops(w.n[0], w.n[1]);
+  }
+
+  // accumulate for the general model (g4acc_accumulate_general): complex M and sign, and the
+  // vertices' left labels when the model is non-density-density.
+  template <class MatrixPair, class Configuration, class Sign>
+  double accumulate_general(const MatrixPair& M_pair, const std::array<Configuration, 2>& configs,
+                            const Sign sign, int memory = G4ACC_HOST) {
+    using Scalar = std::remove_cv_t<std::remove_pointer_t<decltype(M_pair[0].ptr())>>;
+    if (is_complex<Scalar>::value && !model_.complex_m)
+      throw std::logic_error("g4acc: complex M needs an accumulator built with complex_m");
+    g4acc_walker_general w{};
+    for (int s = 0; s < 2; ++s) {
+      const auto& conf = configs[s];
+      const int n = static_cast<int>(conf.size());
+      if (model_.non_density_density)
+        extract_configuration_general(conf, tau_[s], site_[s], band_[s], site_l_[s], band_l_[s]);
+      else
+        extract_configuration(conf, tau_[s], site_[s], band_[s]);
+      w.n[s] = n;
+      w.ld[s] = n ? M_pair[s].leadingDimension() : 1;
+      if (model_.complex_m && !is_complex<Scalar>::value && n) {
+        // Real M on a complex handle: widened into a complex copy (column-major, leading dimension n).
+        mc_[s].assign(2 * static_cast<std::size_t>(n) * n, 0.);
+        const Scalar* m = M_pair[s].ptr();
+        for (int j = 0; j < n; ++j)
+          for (int i = 0; i < n; ++i)
+            mc_[s][2 * (i + static_cast<std::size_t>(n) * j)] = static_cast<double>(m[i + static_cast<std::size_t>(w.ld[s]) * j]);
+        if (memory != G4ACC_HOST)
+          throw std::logic_error("g4acc: a complex handle takes real device M only as complex entries");
+        w.m[s] = mc_[s].data();
+        w.ld[s] = n;
+      }
+      else
+        w.m[s] = n ? reinterpret_cast<const double*>(M_pair[s].ptr()) : nullptr;
+      w.tau[s] = tau_[s].data();
+      w.site[s] = site_[s].data();
+      w.band[s] = band_[s].data();
+      w.site_l[s] = model_.non_density_density ? site_l_[s].data() : nullptr;
+      w.band_l[s] = model_.non_density_density ? band_l_[s].data() : nullptr;
+    }
+    const std::complex<double> z(sign);
+    w.sign[0] = z.real();
+    w.sign[1] = z.imag();
+    const int kind = memory == G4ACC_HOST ? G4ACC_HOST : G4ACC_DEVICE_M;
+    check(g4acc_accumulate_general(handle_, &w, 1, kind));
     return flops(w.n[0], w.n[1]);
   }
 
@@ -235,9 +321,10 @@ private:
 
   Geometry geometry_;
   bool queued_ = false;
+  g4acc_model model_{1, 0, 0};
   g4acc_handle* handle_ = nullptr;
-  std::array<std::vector<double>, 2> tau_;
-  std::array<std::vector<int>, 2> site_, band_;
+  std::array<std::vector<double>, 2> tau_, mc_;
+  std::array<std::vector<int>, 2> site_, band_, site_l_, band_l_;
   std::vector<TpGreensFunction> G4_;
 };
 

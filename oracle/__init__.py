@@ -52,6 +52,12 @@ def lib():
         L.tpo_accumulate.restype = ctypes.c_double
         L.tpo_accumulate.argtypes = [vp, arr_i, arr_pd, arr_i, arr_pd, arr_pd, arr_pd, ctypes.c_double]
         L.tpo_compute_G.argtypes = [vp, arr_i, arr_pd, arr_i, arr_pd, arr_pd, arr_pd]
+        L.tpo_set_spin_symmetric.argtypes = [vp, ctypes.c_int]
+        L.tpo_accumulate_general.restype = ctypes.c_double
+        L.tpo_accumulate_general.argtypes = [vp, arr_i, arr_pd, ctypes.c_int, arr_i, arr_pd, arr_pd, arr_pd,
+                                             arr_pd, arr_pd, ctypes.c_double, ctypes.c_double]
+        L.tpo_compute_G_general.argtypes = [vp, arr_i, arr_pd, ctypes.c_int, arr_i, arr_pd, arr_pd, arr_pd,
+                                            arr_pd, arr_pd]
         for f in (L.tpo_get_Mrrww, L.tpo_get_G):
             f.restype = ctypes.c_longlong
             f.argtypes = [vp, ctypes.c_void_p, ctypes.c_longlong]
@@ -75,6 +81,12 @@ def _ptr(a, t):
     return a.ctypes.data_as(ctypes.POINTER(t))
 
 
+def _is_general(meas) -> bool:
+    """A measurement of the general model: complex M or sign, or vertices with left labels."""
+    return (getattr(meas, "site_l", None) is not None or any(np.iscomplexobj(x) for x in meas.M)
+            or isinstance(meas.sign, complex))
+
+
 BLAS_CANDIDATES = ("/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so", "libopenblas.so")
 
 
@@ -94,7 +106,7 @@ def use_blas(enable: bool = True) -> str:
 class OracleAccumulator:
     """FP64 CPU TpAccumulator restatement (same method names as the reference class)."""
 
-    def __init__(self, geom):
+    def __init__(self, geom, spin_symmetric: bool = True):
         self.geom = geom
         self._keep = [
             np.ascontiguousarray(geom.w_ex, np.int32),
@@ -121,6 +133,7 @@ class OracleAccumulator:
         self._h = lib().tpo_create(ctypes.byref(g), err, 256)
         if not self._h:
             raise ValueError(err.value.decode())
+        lib().tpo_set_spin_symmetric(self._h, 1 if spin_symmetric else 0)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -151,14 +164,51 @@ class OracleAccumulator:
             m[s], tau[s], site[s], band[s] = a.ctypes.data, t.ctypes.data, r.ctypes.data, b.ctypes.data
         return keep, (n, m, ld, tau, site, band)
 
+    def _general_args(self, meas):
+        """Arguments of the general entry points: complex M when any sector is complex, the left
+        labels (site_l, band_l) when the measurement carries them."""
+        keep = []
+        cplx = any(np.iscomplexobj(x) for x in meas.M)
+        n = (ctypes.c_int * 2)(*meas.n)
+        ld = (ctypes.c_int * 2)(*[max(1, x) for x in meas.n])
+        m = (ctypes.c_void_p * 2)()
+        tau = (ctypes.c_void_p * 2)()
+        site = (ctypes.c_void_p * 2)()
+        band = (ctypes.c_void_p * 2)()
+        nd = getattr(meas, "site_l", None) is not None
+        site_l = (ctypes.c_void_p * 2)() if nd else None
+        band_l = (ctypes.c_void_p * 2)() if nd else None
+        for s in range(2):
+            a = np.asfortranarray(meas.M[s], np.complex128 if cplx else np.float64)
+            t = np.ascontiguousarray(meas.tau[s], np.float64)
+            r = np.ascontiguousarray(meas.site[s], np.int32)
+            b = np.ascontiguousarray(meas.band[s], np.int32)
+            keep += [a, t, r, b]
+            m[s], tau[s], site[s], band[s] = a.ctypes.data, t.ctypes.data, r.ctypes.data, b.ctypes.data
+            if nd:
+                rl = np.ascontiguousarray(meas.site_l[s], np.int32)
+                bl = np.ascontiguousarray(meas.band_l[s], np.int32)
+                keep += [rl, bl]
+                site_l[s], band_l[s] = rl.ctypes.data, bl.ctypes.data
+        return keep, (n, m, int(cplx), ld, tau, site, band, site_l, band_l)
+
     def accumulate(self, meas) -> float:
-        keep, (n, m, ld, tau, site, band) = self._args(meas)
-        f = lib().tpo_accumulate(self._h, n, m, ld, tau, site, band, float(meas.sign))
+        if _is_general(meas):
+            keep, args = self._general_args(meas)
+            sign = complex(meas.sign)
+            f = lib().tpo_accumulate_general(self._h, *args, sign.real, sign.imag)
+        else:
+            keep, (n, m, ld, tau, site, band) = self._args(meas)
+            f = lib().tpo_accumulate(self._h, n, m, ld, tau, site, band, float(meas.sign))
         if f < 0:
             raise NotImplementedError("Specified four point type not implemented.")
         return f
 
     def compute_G(self, meas):
+        if _is_general(meas):
+            keep, args = self._general_args(meas)
+            lib().tpo_compute_G_general(self._h, *args)
+            return
         keep, (n, m, ld, tau, site, band) = self._args(meas)
         lib().tpo_compute_G(self._h, n, m, ld, tau, site, band)
 

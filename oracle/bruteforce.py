@@ -20,8 +20,17 @@ import numpy as np
 from dca_amd import geometry as geo
 
 
+def column_labels(meas, spin: int):
+    """(sites, bands) that label the columns of M: the vertices' left labels for non-density-density
+    vertices (M is an inverse matrix, cached_ndft_base.hpp:97-103), else the same as the rows'."""
+    if getattr(meas, "site_l", None) is not None:
+        return meas.site_l[spin], meas.band_l[spin]
+    return meas.site[spin], meas.band[spin]
+
+
 def ndft_direct(geom: geo.Geometry, meas, spin: int) -> np.ndarray:
-    """Returns f[r1, r2, b1, b2, w1E, w2E] by the defining double sum."""
+    """Returns f[r1, r2, b1, b2, w1E, w2E] by the defining double sum (rows of M at the vertices'
+    right labels, columns at their left labels; M real or complex)."""
     nb, nc, nE = geom.nb, geom.nc, geom.n_ext
     w = geo.vertex_frequencies(nE, geom.beta)
     n = meas.n[spin]
@@ -31,9 +40,10 @@ def ndft_direct(geom: geo.Geometry, meas, spin: int) -> np.ndarray:
     tau = meas.tau[spin]
     ph = np.exp(1j * np.outer(tau, w))  # (i, w)
     M = meas.M[spin]
+    site_c, band_c = column_labels(meas, spin)
     for i in range(n):
         for j in range(n):
-            out[meas.site[spin][i], meas.site[spin][j], meas.band[spin][i], meas.band[spin][j]] += \
+            out[meas.site[spin][i], site_c[j], meas.band[spin][i], band_c[j]] += \
                 M[i, j] * np.outer(ph[i], ph[j].conj())
     return out
 
@@ -49,13 +59,17 @@ def mtilde_direct(geom: geo.Geometry, meas, spin: int) -> np.ndarray:
         return out
     T = geom.cluster.dft  # (k, r)
     tau = meas.tau[spin]
-    site, band = meas.site[spin], meas.band[spin]
-    # Phi[b, k, w, i] = [b_i = b] exp(i k.r_i) exp(i w tau_i)
-    phi = np.zeros((nb, nc, nE, n), complex)
-    for i in range(n):
-        phi[band[i], :, :, i] = np.outer(T[:, site[i]], np.exp(1j * w * tau[i]))
-    phi2 = phi.reshape(nb * nc * nE, n)
-    mt = phi2 @ meas.M[spin] @ phi2.conj().T / nc  # rows (b1,k1,w1) C-order, cols (b2,k2,w2)
+
+    def phi_of(site, band):
+        # Phi[b, k, w, i] = [b_i = b] exp(i k.r_i) exp(i w tau_i)
+        phi = np.zeros((nb, nc, nE, n), complex)
+        for i in range(n):
+            phi[band[i], :, :, i] = np.outer(T[:, site[i]], np.exp(1j * w * tau[i]))
+        return phi.reshape(nb * nc * nE, n)
+
+    rows = phi_of(meas.site[spin], meas.band[spin])
+    cols = phi_of(*column_labels(meas, spin))
+    mt = rows @ meas.M[spin] @ cols.conj().T / nc  # rows (b1,k1,w1) C-order, cols (b2,k2,w2)
     mt = mt.reshape(nb, nc, nE, nb, nc, nE)
     return np.transpose(mt, (0, 3, 1, 4, 2, 5))
 
@@ -145,7 +159,9 @@ def g_single_vertex(geom: geo.Geometry, vertices) -> np.ndarray:
       G_s(b1, b2) = beta G0_s(b1,b2;p1) delta_{p1p2}
                     - (m / Nc) G0_s(b1,b;p1) exp(i(k1.r + w1 tau)) G0_s(b,b2;p2) exp(-i(k2.r + w2 tau)),
     a diagonal plus a rank-one matrix that is not symmetric, so every off-diagonal G read of the
-    G4 update sees its argument order. vertices[s] = (tau, site, band, m) or None."""
+    G4 update sees its argument order. vertices[s] = (tau, site, band, m) or None, or
+    (tau, site, band, m, site_l, band_l) for a non-density-density vertex, whose column factor
+    G0_s(b_l, b2; p2) exp(-i(k2.r_l + w2 tau)) carries its left labels; m may be complex."""
     nb, nc, nE = geom.nb, geom.nc, geom.n_ext
     w = geo.vertex_frequencies(nE, geom.beta)
     G = np.zeros((nb, nb, 2, nc, nc, nE, nE), complex)
@@ -156,16 +172,20 @@ def g_single_vertex(geom: geo.Geometry, vertices) -> np.ndarray:
                 G[:, :, s, k, k, i, i] += geom.beta * g0[:, :, k, i]
         if vertices[s] is None:
             continue
-        tau, site, band, m = vertices[s]
-        ph = np.exp(1j * (geom.cluster.k @ geom.cluster.r[site])[:, None] + 1j * w[None, :] * tau)  # (k, w)
-        left = g0[:, band] * ph[None]              # (b1, k1, w1)
-        right = g0[band, :] * np.conj(ph)[None]    # (b2, k2, w2)
+        tau, site, band, m = vertices[s][:4]
+        site_l, band_l = vertices[s][4:6] if len(vertices[s]) > 4 else (site, band)
+
+        def phase(r):
+            return np.exp(1j * (geom.cluster.k @ geom.cluster.r[r])[:, None] + 1j * w[None, :] * tau)  # (k, w)
+
+        left = g0[:, band] * phase(site)[None]                 # (b1, k1, w1)
+        right = g0[band_l, :] * np.conj(phase(site_l))[None]   # (b2, k2, w2)
         G[:, :, s] -= (m / nc) * np.einsum("akw,blv->abklwv", left, right)
     return G
 
 
-def g4_definition(geom: geo.Geometry, G: np.ndarray, channel: int, sign: float,
-                  variant: str = "") -> np.ndarray:
+def g4_definition(geom: geo.Geometry, G: np.ndarray, channel: int, sign: complex,
+                  variant: str = "", spin_symmetric: bool = True) -> np.ndarray:
     """G4 of one measurement from its G (reference layout), by the channel definitions, shaped
     like g4_free. With L_s(b, b'; p1, p2) = G(b', b, s, k1, k2, w1 + ext, w2 + ext) (getGMultiband's
     band-transposed read, tp_accumulator_cpu.hpp:388-405), a = sign / 2, p + Q = (k + q, w + v) and
@@ -184,10 +204,19 @@ def g4_definition(geom: geo.Geometry, G: np.ndarray, channel: int, sign: float,
     contracts G(k1, k2, s) G(q-k1, q-k2, -s), which is what is written here. The band slots of the
     several-band terms are those of the code (:513-523, 532-543, 561-572, 640-648).
 
+    Models that are not spin symmetric (spin_symmetric=False; the spin is part of the band index)
+    read spin sector 0 only, with the whole sign and no 1/2 (tp_accumulator_cpu.hpp:668-757):
+      PP     : sign [L_0(b1,b3; p1, p2) L_0(b2,b4; Q-p1, Q-p2) - L_0(b1,b4; p1, Q-p2) L_0(b2,b3; Q-p1, p2)]
+      MAG    : sign [L_0(b2,b1; p1, p1+Q) L_0(b3,b4; p2+Q, p2) - L_0(b3,b1; p2+Q, p1+Q) L_0(b2,b4; p1, p2)]
+    the Wick contractions of <c+(k1+q, s1) c(k1, s2) c+(k2, s4) c(k2+q, s3)> written at :711-714 and
+    their pp counterpart (:681-683, :694-695; as in the symmetric pp term, the code's argument
+    order). The other channels are not defined there (:752-753).
+
     variant (test-only planted mistakes, to show that the checks see them): "pp_comment" (the pp
     comment's argument order), "crossed_transposed" (G(p1+Q, p2+Q) in the crossed ph term),
     "transverse_same_spin" (s instead of -s), "mag_spin_sum" (no sigma weights in the magnetic
-    direct term)."""
+    direct term), "nss_pp_exchange_slots" (b3 <-> b4 in the exchange term of the non-symmetric pp),
+    "nss_half" (the symmetric branch's 1/2 in the non-symmetric branch)."""
     nb, nc, nw = geom.nb, geom.nc, geom.n_freq
     ext = geom.ext
     add, sub = geom.cluster.add, geom.cluster.sub
@@ -204,6 +233,25 @@ def g4_definition(geom: geo.Geometry, G: np.ndarray, channel: int, sign: float,
         return L[s][:, :, kA, kB, wA + ext, wB + ext]  # (b, b', k1, w1, k2, w2)
 
     out = np.zeros(geom.g4_shape(), complex)
+    if not spin_symmetric:
+        f = sign * (0.5 if variant == "nss_half" else 1.0)
+        for iv, nu in enumerate(geom.w_ex):
+            for iq, q in enumerate(geom.k_ex):
+                K1q, K2q = add[K1, q], add[K2, q]
+                mK1, mK2 = sub[K1, q], sub[K2, q]
+                mW1, mW2 = nu + nw - 1 - W1, nu + nw - 1 - W2
+                if channel == geo.PARTICLE_PARTICLE_UP_DOWN:
+                    val = np.einsum("ac...,bd...->abcd...", read(0, K1, K2, W1, W2), read(0, mK1, mK2, mW1, mW2))
+                    ex = "ac...,bd...->abcd..." if variant == "nss_pp_exchange_slots" else "ad...,bc...->abcd..."
+                    val = val - np.einsum(ex, read(0, K1, mK2, W1, mW2), read(0, mK1, K2, mW1, W2))
+                elif channel == geo.PARTICLE_HOLE_MAGNETIC:
+                    val = np.einsum("ba...,cd...->abcd...", read(0, K1, K1q, W1, W1 + nu), read(0, K2q, K2, W2 + nu, W2))
+                    val = val - np.einsum("ca...,bd...->abcd...", read(0, K2q, K1q, W2 + nu, W1 + nu),
+                                          read(0, K1, K2, W1, W2))
+                else:
+                    raise ValueError("channel not defined for models that are not spin symmetric")
+                out[..., iq, iv] = f * val
+        return out
     for iv, nu in enumerate(geom.w_ex):
         for iq, q in enumerate(geom.k_ex):
             K1q, K2q = add[K1, q], add[K2, q]

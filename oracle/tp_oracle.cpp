@@ -100,6 +100,8 @@ struct tpo_acc {
   std::vector<std::vector<cd>> G4;
   CMat Ga, Gb, G0M;           // band-block work spaces (tp_accumulator_cpu.hpp:208)
   bool keep_mrrww = true;     // copy M_r_r_w_w for tpo_get_Mrrww (a test hook; off for timing)
+  // Parameters::model_type::lattice_type::spin_symmetric (tp_accumulator_base.hpp:167).
+  bool spin_symmetric = true;
 
   size_t g0i(int b1, int b2, int s, int k, int w) const {
     return b1 + nb * (b2 + nb * (s + 2 * (k + static_cast<size_t>(nc) * w)));
@@ -131,21 +133,38 @@ struct tpo_acc {
   // CachedNdft<CPU>::execute (tp/ndft/cached_ndft_cpu.hpp:106-140) with sortConfiguration
   // (tp/ndft/cached_ndft_base.hpp:89-129), computeT (:142-158), computeMMatrix (:160-174),
   // computeTSubmatrices (:176-197), executeTrimmedFT (:200-222), copyPartialResult (:239-252).
-  // Density-density vertices: left and right orbital are the same (cached_ndft_base.hpp:72-78).
-  double ndft(int spin, int n_v, const double* m, int ld, const double* tau, const int* site,
-              const int* band, std::vector<cd>& out) {
+  // The rows of M are sorted by the vertices' right orbitals and the columns by their left ones
+  // ("M is an inverse matrix", cached_ndft_base.hpp:97-103); site_l / band_l == nullptr is the
+  // density-density case, where both sides are the same sort (:72-78). MS: double or complex.
+  template <typename MS>
+  double ndft(int spin, int n_v, const MS* m, int ld, const double* tau, const int* site,
+              const int* band, const int* site_l, const int* band_l, std::vector<cd>& out) {
     const int n_orb = nb * nc;
-    std::vector<Triple> cfg(n_v);
+    const bool nd = site_l != nullptr;
+    std::vector<Triple> cfg_l(n_v), cfg_r;
     for (int l = 0; l < n_v; ++l)
-      cfg[l] = Triple{band[l] + nb * site[l], l, tau[l]};
-    std::sort(cfg.begin(), cfg.end());
-    std::vector<int> start(n_orb), end(n_orb);
-    for (int orb = 0; orb < n_orb; ++orb) {
-      Triple t{orb, 0, 0};
-      start[orb] = std::lower_bound(cfg.begin(), cfg.end(), t) - cfg.begin();
-      t.idx = n_v;
-      end[orb] = std::upper_bound(cfg.begin(), cfg.end(), t) - cfg.begin();
+      cfg_l[l] = Triple{band[l] + nb * site[l], l, tau[l]};
+    std::sort(cfg_l.begin(), cfg_l.end());
+    if (nd) {
+      cfg_r.resize(n_v);
+      for (int l = 0; l < n_v; ++l)
+        cfg_r[l] = Triple{band_l[l] + nb * site_l[l], l, tau[l]};
+      std::sort(cfg_r.begin(), cfg_r.end());
     }
+    const std::vector<Triple>& cfg_right = nd ? cfg_r : cfg_l;
+    auto ranges = [&](const std::vector<Triple>& cfg, std::vector<int>& start, std::vector<int>& end) {
+      start.resize(n_orb);
+      end.resize(n_orb);
+      for (int orb = 0; orb < n_orb; ++orb) {
+        Triple t{orb, 0, 0};
+        start[orb] = std::lower_bound(cfg.begin(), cfg.end(), t) - cfg.begin();
+        t.idx = n_v;
+        end[orb] = std::upper_bound(cfg.begin(), cfg.end(), t) - cfg.begin();
+      }
+    };
+    std::vector<int> start_l, end_l, start_r, end_r;
+    ranges(cfg_l, start_l, end_l);
+    ranges(cfg_right, start_r, end_r);
 
     CMat T;
     T.resize(nwE, n_v);
@@ -158,23 +177,23 @@ struct tpo_acc {
     double flops = 0;
     CMat Mij, Tl, Tr, TlM, TlMTr;
     for (int orb_j = 0; orb_j < n_orb; ++orb_j) {
-      const int n_j = end[orb_j] - start[orb_j];
+      const int n_j = end_r[orb_j] - start_r[orb_j];
       for (int orb_i = 0; orb_i < n_orb; ++orb_i) {
-        const int n_i = end[orb_i] - start[orb_i];
+        const int n_i = end_l[orb_i] - start_l[orb_i];
         const int b1 = orb_i % nb, r1 = orb_i / nb;
         const int b2 = orb_j % nb, r2 = orb_j / nb;
         if (n_i > 0 && n_j > 0) {
           Mij.resize(n_i, n_j);
-          for (int lj = start[orb_j]; lj < end[orb_j]; ++lj)
-            for (int li = start[orb_i]; li < end[orb_i]; ++li)
-              Mij(li - start[orb_i], lj - start[orb_j]) =
-                  m[cfg[li].idx + static_cast<size_t>(ld) * cfg[lj].idx];
+          for (int lj = start_r[orb_j]; lj < end_r[orb_j]; ++lj)
+            for (int li = start_l[orb_i]; li < end_l[orb_i]; ++li)
+              Mij(li - start_l[orb_i], lj - start_r[orb_j]) =
+                  cd(m[cfg_l[li].idx + static_cast<size_t>(ld) * cfg_right[lj].idx]);
           Tl.resize(nwE, n_i);
-          for (int li = start[orb_i]; li < end[orb_i]; ++li)
-            std::copy_n(&T(0, cfg[li].idx), nwE, &Tl(0, li - start[orb_i]));
+          for (int li = start_l[orb_i]; li < end_l[orb_i]; ++li)
+            std::copy_n(&T(0, cfg_l[li].idx), nwE, &Tl(0, li - start_l[orb_i]));
           Tr.resize(nwE, n_j);
-          for (int lj = start[orb_j]; lj < end[orb_j]; ++lj)
-            std::copy_n(&T(0, cfg[lj].idx), nwE, &Tr(0, lj - start[orb_j]));
+          for (int lj = start_r[orb_j]; lj < end_r[orb_j]; ++lj)
+            std::copy_n(&T(0, cfg_right[lj].idx), nwE, &Tr(0, lj - start_r[orb_j]));
 
           TlM.resize(nwE, n_j);
           zgemm('N', nwE, n_j, n_i, 1., Tl.data(), nwE, Mij.data(), n_i, 0., TlM.data(), nwE);
@@ -264,7 +283,7 @@ struct tpo_acc {
 
   // updateG4Atomic (tp_accumulator_cpu.hpp:759-810).
   void atomic_update(cd* g4, int s_a, int k1_a, int k2_a, int w1_a, int w2_a, int s_b, int k1_b,
-                     int k2_b, int w1_b, int w2_b, double alpha, bool cross_legs) {
+                     int k2_b, int w1_b, int w2_b, cd alpha, bool cross_legs) {
     if (nb == 1) {
       const cd a = getG1(s_a, k1_a, k2_a, w1_a, w2_a);
       const cd b = getG1(s_b, k1_b, k2_b, w1_b, w2_b);
@@ -278,14 +297,14 @@ struct tpo_acc {
         for (int b2 = 0; b2 < nb; ++b2)
           for (int b1 = 0; b1 < nb; ++b1) {
             const cd prod = cross_legs ? Ga(b1, b4) * Gb(b2, b3) : Ga(b1, b3) * Gb(b2, b4);
-            *g4 += cd(alpha) * prod;
+            *g4 += alpha * prod;
             ++g4;
           }
   }
 
   // updateG4SpinDifference (tp_accumulator_cpu.hpp:812-869).
   void spin_difference_update(cd* g4, double sign, int k1_a, int k2_a, int w1_a, int w2_a,
-                              int k1_b, int k2_b, int w1_b, int w2_b, double alpha,
+                              int k1_b, int k2_b, int w1_b, int w2_b, cd alpha,
                               bool cross_legs) {
     if (nb == 1) {
       *g4 += alpha * (getG1(0, k1_a, k2_a, w1_a, w2_a) + cd(sign) * getG1(1, k1_a, k2_a, w1_a, w2_a)) *
@@ -305,10 +324,13 @@ struct tpo_acc {
           }
   }
 
-  // updateG4, spin-symmetric branch (tp_accumulator_cpu.hpp:407-666). Returns false for a
-  // channel the reference does not implement (:663-664).
-  bool update_G4(int channel_id, double factor, double& flops) {
-    const double sign_over_2 = 0.5 * factor;
+  // updateG4, spin-symmetric branch (tp_accumulator_cpu.hpp:407-666); the factor is the sign
+  // (SignType: +-1, or a complex phase for complex models, :433-438). Returns false for a channel
+  // the reference does not implement (:663-664, :752-753).
+  bool update_G4(int channel_id, cd factor, double& flops) {
+    if (!spin_symmetric)
+      return update_G4_no_spin(channel_id, factor, flops);
+    const cd sign_over_2 = 0.5 * factor;
     auto momentum_sum = [&](int k, int q) { return kadd[k + nc * q]; };
     auto q_minus_k = [&](int k, int q) { return ksub[k + nc * q]; };
     auto w_plus_w_ex = [](int w, int w_ex) { return w + w_ex; };
@@ -438,16 +460,108 @@ struct tpo_acc {
     }
   }
 
+  // updateG4, branch of models that are not spin symmetric (tp_accumulator_cpu.hpp:668-757): the
+  // spin is part of the band index, only spin sector 0 of G is read, and there is no 1/2.
+  bool update_G4_no_spin(int channel_id, cd factor, double& flops) {
+    auto q_minus_k = [&](int k, int q) { return ksub[k + nc * q]; };
+    auto q_plus_k = [&](int k, int q) { return kadd[k + nc * q]; };
+    auto w_plus_w_ex = [](int w, int w_ex) { return w + w_ex; };
+    const int nw_c = nw;
+    auto w_ex_minus_w = [nw_c](int w, int w_ex) { return w_ex + nw_c - 1 - w; };
+    const double f_atomic = 3 * std::pow(nb, 4);
+    const double n_loops = double(nw) * nw * nc * nc;
+    std::vector<cd>& g4 = G4[channel_id];
+    Ga.resize(nb, nb);
+    Gb.resize(nb, nb);
+    switch (channels[channel_id]) {
+      case PP_UP_DOWN:
+        for (int iv = 0; iv < nv; ++iv) {
+          const int w_ex = wex[iv];
+          for (int iq = 0; iq < nq; ++iq) {
+            const int k_ex = kex[iq];
+            for (int w2 = 0; w2 < nw; ++w2)
+              for (int k2 = 0; k2 < nc; ++k2)
+                for (int w1 = 0; w1 < nw; ++w1)
+                  for (int k1 = 0; k1 < nc; ++k1) {
+                    // G(k2, k1, b1, b3) * G(k_ex - k2, k_ex - k1, b2, b4)
+                    getGM(0, k1, k2, w1, w2, Ga);
+                    getGM(0, q_minus_k(k1, k_ex), q_minus_k(k2, k_ex), w_ex_minus_w(w1, w_ex),
+                          w_ex_minus_w(w2, w_ex), Gb);
+                    for (int b4 = 0; b4 < nb; ++b4)
+                      for (int b3 = 0; b3 < nb; ++b3)
+                        for (int b2 = 0; b2 < nb; ++b2)
+                          for (int b1 = 0; b1 < nb; ++b1)
+                            g4[g4i(b1, b2, b3, b4, k1, w1, k2, w2, iq, iv)] += factor * Ga(b1, b3) * Gb(b2, b4);
+                    // -G(k1, k_ex - k2, b1, b4) * G(k_ex - k1, k2, b2, b3)
+                    getGM(0, k1, q_minus_k(k2, k_ex), w1, w_ex_minus_w(w2, w_ex), Ga);
+                    getGM(0, q_minus_k(k1, k_ex), k2, w_ex_minus_w(w1, w_ex), w2, Gb);
+                    for (int b4 = 0; b4 < nb; ++b4)
+                      for (int b3 = 0; b3 < nb; ++b3)
+                        for (int b2 = 0; b2 < nb; ++b2)
+                          for (int b1 = 0; b1 < nb; ++b1)
+                            g4[g4i(b1, b2, b3, b4, k1, w1, k2, w2, iq, iv)] -= factor * Ga(b1, b4) * Gb(b2, b3);
+                  }
+          }
+        }
+        flops += n_loops * 4 * f_atomic;
+        return true;
+
+      case PH_MAGNETIC:
+        for (int iv = 0; iv < nv; ++iv) {
+          const int w_ex = wex[iv];
+          for (int iq = 0; iq < nq; ++iq) {
+            const int k_ex = kex[iq];
+            for (int w2 = 0; w2 < nw; ++w2)
+              for (int k2 = 0; k2 < nc; ++k2)
+                for (int w1 = 0; w1 < nw; ++w1)
+                  for (int k1 = 0; k1 < nc; ++k1) {
+                    // G(k1, k1+k_ex, s2, s1) G(k2+k_ex, k2, s3, s4)
+                    getGM(0, k1, q_plus_k(k1, k_ex), w1, w_plus_w_ex(w1, w_ex), Ga);
+                    getGM(0, q_plus_k(k2, k_ex), k2, w_plus_w_ex(w2, w_ex), w2, Gb);
+                    for (int b4 = 0; b4 < nb; ++b4)
+                      for (int b3 = 0; b3 < nb; ++b3)
+                        for (int b2 = 0; b2 < nb; ++b2)
+                          for (int b1 = 0; b1 < nb; ++b1)
+                            g4[g4i(b1, b2, b3, b4, k1, w1, k2, w2, iq, iv)] += factor * Ga(b2, b1) * Gb(b3, b4);
+                    // -G(k2+k_ex, k1+k_ex, s3, s1) G(k1, k2, s2, s4)
+                    getGM(0, q_plus_k(k2, k_ex), q_plus_k(k1, k_ex), w_plus_w_ex(w2, w_ex),
+                          w_plus_w_ex(w1, w_ex), Ga);
+                    getGM(0, k1, k2, w1, w2, Gb);
+                    for (int b4 = 0; b4 < nb; ++b4)
+                      for (int b3 = 0; b3 < nb; ++b3)
+                        for (int b2 = 0; b2 < nb; ++b2)
+                          for (int b1 = 0; b1 < nb; ++b1)
+                            g4[g4i(b1, b2, b3, b4, k1, w1, k2, w2, iq, iv)] -= factor * Ga(b3, b1) * Gb(b2, b4);
+                  }
+          }
+        }
+        flops += n_loops * std::pow(nb, 4) * 2 * 3;
+        return true;
+
+      default:
+        return false;
+    }
+  }
+
   // computeM (tp_accumulator_cpu.hpp:254-276) followed by computeG (:278-286).
-  double compute_M_and_G(const int n[2], const double* const m[2], const int ld[2],
+  // m[s]: real M, or interleaved complex M when m_complex; site_l / band_l: the vertices' left
+  // labels (nullptr: density-density vertices).
+  double compute_M_and_G(const int n[2], const double* const m[2], bool m_complex, const int ld[2],
                          const double* const tau[2], const int* const site[2],
-                         const int* const band[2]) {
+                         const int* const band[2], const int* const* site_l = nullptr,
+                         const int* const* band_l = nullptr) {
     double flops = 0;
     std::vector<cd> M_r_r_w_w(static_cast<size_t>(nc) * nc * nb * nb * 2 * nwE * nwE, cd(0));
     for (int s = 0; s < 2; ++s) {
       if (!n[s])
         continue;
-      flops += ndft(s, n[s], m[s], ld[s], tau[s], site[s], band[s], M_r_r_w_w);
+      const int* sl = site_l ? site_l[s] : nullptr;
+      const int* bl = band_l ? band_l[s] : nullptr;
+      if (m_complex)
+        flops += ndft(s, n[s], reinterpret_cast<const cd*>(m[s]), ld[s], tau[s], site[s], band[s], sl, bl,
+                      M_r_r_w_w);
+      else
+        flops += ndft(s, n[s], m[s], ld[s], tau[s], site[s], band[s], sl, bl, M_r_r_w_w);
     }
     if (keep_mrrww)
       Mrrww = M_r_r_w_w;
@@ -561,10 +675,10 @@ double tpo_accumulate(tpo_acc* a, const int n[2], const double* const m[2], cons
   double flops = 0;
   if (!(n[0] + n[1]))
     return flops;
-  flops += a->compute_M_and_G(n, m, ld, tau, site, band);
+  flops += a->compute_M_and_G(n, m, false, ld, tau, site, band);
   for (int c = 0; c < static_cast<int>(a->G4.size()); ++c) {
     double f = 0;
-    if (!a->update_G4(c, sign, f))
+    if (!a->update_G4(c, cd(sign), f))
       return -1;
     flops += 1e-9 * f;
   }
@@ -573,7 +687,35 @@ double tpo_accumulate(tpo_acc* a, const int n[2], const double* const m[2], cons
 
 void tpo_compute_G(tpo_acc* a, const int n[2], const double* const m[2], const int ld[2],
                    const double* const tau[2], const int* const site[2], const int* const band[2]) {
-  a->compute_M_and_G(n, m, ld, tau, site, band);
+  a->compute_M_and_G(n, m, false, ld, tau, site, band);
+}
+
+void tpo_set_spin_symmetric(tpo_acc* a, int spin_symmetric) { a->spin_symmetric = spin_symmetric != 0; }
+
+// accumulate (tp_accumulator_cpu.hpp:231-252) for the general model: complex M, the vertices'
+// right (site, band) and left (site_l, band_l) labels, a complex sign.
+double tpo_accumulate_general(tpo_acc* a, const int n[2], const double* const m[2], int m_complex,
+                              const int ld[2], const double* const tau[2], const int* const site[2],
+                              const int* const band[2], const int* const site_l[2],
+                              const int* const band_l[2], double sign_re, double sign_im) {
+  double flops = 0;
+  if (!(n[0] + n[1]))
+    return flops;
+  flops += a->compute_M_and_G(n, m, m_complex != 0, ld, tau, site, band, site_l, band_l);
+  for (int c = 0; c < static_cast<int>(a->G4.size()); ++c) {
+    double f = 0;
+    if (!a->update_G4(c, cd(sign_re, sign_im), f))
+      return -1;
+    flops += 1e-9 * f;
+  }
+  return flops;
+}
+
+void tpo_compute_G_general(tpo_acc* a, const int n[2], const double* const m[2], int m_complex,
+                           const int ld[2], const double* const tau[2], const int* const site[2],
+                           const int* const band[2], const int* const site_l[2],
+                           const int* const band_l[2]) {
+  a->compute_M_and_G(n, m, m_complex != 0, ld, tau, site, band, site_l, band_l);
 }
 
 static long long copy_out(const std::vector<cd>& v, double* out, long long n_complex) {

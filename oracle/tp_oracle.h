@@ -20,7 +20,7 @@
  *   G0   (b1, b2, s, k, wE)                 wE over the EXTENDED vertex frequencies
  *   G    (b1, b2, s, k1, k2, w1E, w2E)      the reference's TpAccumulatorBase::G_
  *   G4   (b1, b2, b3, b4, k1, w1, k2, w2, k_ex, w_ex) per channel, w over COMPACT frequencies
- *   M    column-major N x N real matrix per spin (leading dimension ld)
+ *   M    column-major N x N real (or complex) matrix per spin (leading dimension ld)
  * Complex arrays are interleaved (re, im) doubles.
  */
 #ifndef TP_ORACLE_H
@@ -68,6 +68,23 @@ void tpo_reset(tpo_acc* a);
 double tpo_accumulate(tpo_acc* a, const int n[2], const double* const m[2], const int ld[2],
                       const double* const tau[2], const int* const site[2],
                       const int* const band[2], double sign);
+
+/* Parameters::model_type::lattice_type::spin_symmetric (tp_accumulator_base.hpp:167); 0 selects
+ * the non-spin-symmetric updateG4 branch (tp_accumulator_cpu.hpp:668-757: pp and magnetic only,
+ * reading spin sector 0). Default 1. */
+void tpo_set_spin_symmetric(tpo_acc* a, int spin_symmetric);
+
+/* accumulate for the general model: m[s] real (m_complex = 0) or interleaved complex; each vertex
+ * has right labels (site, band) and left labels (site_l, band_l; NULL arrays: the same, i.e.
+ * density-density vertices, cached_ndft_base.hpp:72-78); the sign is a complex phase. */
+double tpo_accumulate_general(tpo_acc* a, const int n[2], const double* const m[2], int m_complex,
+                              const int ld[2], const double* const tau[2], const int* const site[2],
+                              const int* const band[2], const int* const site_l[2],
+                              const int* const band_l[2], double sign_re, double sign_im);
+void tpo_compute_G_general(tpo_acc* a, const int n[2], const double* const m[2], int m_complex,
+                           const int ld[2], const double* const tau[2], const int* const site[2],
+                           const int* const band[2], const int* const site_l[2],
+                           const int* const band_l[2]);
 
 /* Only the M -> G step (computeM + computeG). The result is readable with tpo_get_G. */
 void tpo_compute_G(tpo_acc* a, const int n[2], const double* const m[2], const int ld[2],

@@ -57,6 +57,18 @@ int main() {
     expect(thrown, "a vertex with different left and right orbitals raises std::logic_error");
   }
 
+  {
+    // Models with non-density-density vertices read both label sets.
+    std::vector<int> site_l, band_l;
+    std::vector<Vertex> nd{{0.2, 0, 1, 2, 3}, {0.4, 1, 0, 1, 2}};
+    g4acc::extract_configuration_general(nd, tau, site, band, site_l, band_l);
+    expect(site[0] == 3 && band[0] == 2 && site_l[0] == 1 && band_l[0] == 0 && site[1] == 2 &&
+               band[1] == 1 && site_l[1] == 0 && band_l[1] == 1 && tau[1] == 0.4,
+           "left and right labels of non-density-density vertices");
+    expect(g4acc::is_complex<std::complex<double>>::value && !g4acc::is_complex<double>::value,
+           "complex scalars select the general entry");
+  }
+
   // G0(b1, s1, b2, s2, k, w) = w + 100 k + 1000 b1 + 2000 b2 + 10000 s1 (+ i s2)
   auto g0 = [](int b1, int s1, int b2, int s2, int k, int w) {
     return std::complex<double>(w + 100. * k + 1000. * b1 + 2000. * b2 + 10000. * s1, s2);

@@ -446,6 +446,32 @@ def test_fused_equals_unfused_mtog(name, precision, monkeypatch):
         assert err < (1e-14 if precision == "fp64" else 1e-6)
 
 
+@pytest.mark.parametrize("name", ["1band_6x6", "1band_8x8"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_large_cluster_mtog_equals_fused(name, precision, monkeypatch):
+    """6x6 and 8x8: the matrix-core S3 plus site-transform kernels (default) against the
+    register-fused right kernel (mtog_large = 0) and the four-stage kernels, over batches of
+    two walkers (four slots: the chunked E workspace) with uneven vertex counts."""
+    wl = case(name)
+    ms = wl.draw(2)
+    large = g4acc.TpAccumulator(wl.geometry, precision=precision, max_walkers=2, max_vertices=wl.n_max)
+    reg = g4acc.TpAccumulator(wl.geometry, precision=precision, max_walkers=2, max_vertices=wl.n_max,
+                              tuned=False)
+    reg.set_tuning(mtog_large=0)
+    monkeypatch.setenv("G4ACC_UNFUSED_MTOG", "1")
+    four = g4acc.TpAccumulator(wl.geometry, precision=precision, max_vertices=wl.n_max)
+    tol = 1e-14 if precision == "fp64" else 1e-6
+    for meas in ms:
+        a, b, c = large.compute_G(meas), reg.compute_G(meas), four.compute_G(meas)
+        print(f"{name} {precision} large vs fused {linf(b, a):.3e}, vs four-stage {linf(c, a):.3e}", flush=True)
+        assert linf(b, a) < tol and linf(c, a) < tol
+    for acc in (large, reg):
+        acc.resetAccumulation()
+        acc.accumulate_batch(ms)
+    for x, y in zip(large.get_G4(), reg.get_G4()):
+        assert linf(y, x) < tol
+
+
 @pytest.mark.parametrize("name,dense", [("1band_2x2", False), ("1band_2x2", True),
                                         ("1band_4x4_4q", False), ("1band_8site", False)])
 def test_mfma_right_mtog_matches_fp64(name, dense, monkeypatch):
