@@ -88,6 +88,7 @@ struct g4acc_handle {
   std::vector<void*> d_g4;         // per channel, 16 B per entry
   int *d_perm = nullptr, *d_off = nullptr, *d_err = nullptr;
   void *d_T = nullptr, *d_B = nullptr, *d_A = nullptr, *d_E = nullptr, *d_G = nullptr;
+  void* d_Ms = nullptr;            // M in sorted order (6x6 / 8x8 matrix-core M -> G)
   void* d_UV = nullptr;            // rank-one factors of the tiled single-band G4 kernel
   // Model (g4acc_create_model). The general model (any field off its default) runs the
   // four-stage M -> G and the per-entry G4 kernels.
@@ -471,6 +472,7 @@ MtoGArgs<Real> mtog_args(g4acc_handle* h, int n_slots) {
   a.tw = h->mtog_tw;
   a.mfma = h->mtog_mfma;
   a.large = h->mtog_large != 0;
+  a.Ms = h->d_Ms;
   a.perm = h->d_perm;
   a.off = h->d_off;
   a.T = static_cast<Cplx<Real>*>(h->d_T);
@@ -823,7 +825,11 @@ int g4acc_create_model(const g4acc_geometry* gm, const g4acc_options* opt, const
     h->grid_L = detect_grid(g.nc, gm->dft, xy, kxy_to_k);
   h->large_ok = h->fused_mtog && (g.nc == 36 || g.nc == 64) && h->grid_L * h->grid_L == g.nc;
   h->mtog_large = h->large_ok ? 1 : 0;
-  const size_t per_slot = h->fused_mtog ? cb * D * (NM + (h->large_ok ? D : 0)) : cb * (2 * D * NM + D * D);
+  // Per slot of a chunk: A (D x N); the matrix-core 6x6 / 8x8 path adds B (D x N), E (D x D) and
+  // the sorted M (N x N real); the four-stage path B and E.
+  const size_t per_slot = !h->fused_mtog ? cb * (2 * D * NM + D * D)
+                          : h->large_ok  ? cb * (2 * D * NM + D * D) + cb / 2 * NM * NM
+                                         : cb * D * NM;
   h->chunk = static_cast<int>(std::max<size_t>(1, std::min<size_t>(W2, workspace_budget_bytes() / per_slot)));
   const size_t CH = h->chunk;
 
@@ -835,7 +841,10 @@ int g4acc_create_model(const g4acc_geometry* gm, const g4acc_options* opt, const
       (rc = h->alloc(&h->d_perm, W2 * NM, "perm")) ||
       (rc = h->alloc(&h->d_off, W2 * (g.n_orb + 1), "off")) || (rc = h->alloc(&h->d_err, 1, "err")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_T), cb * W2 * NM * g.nwE, "T")) ||
-      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_B), h->fused_mtog ? 1 : cb * CH * NM * D, "B")) ||
+      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_B),
+                     h->fused_mtog && !h->large_ok ? 1 : cb * CH * NM * D, "B")) ||
+      (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_Ms), h->large_ok ? cb / 2 * CH * NM * NM : 1,
+                     "sorted M")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_A), cb * CH * NM * D, "A")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_E),
                      h->fused_mtog && !h->large_ok ? 1 : cb * CH * D * D, "E")) ||
@@ -1391,6 +1400,7 @@ void launch_mtog_noalloc(g4acc_handle* h, const NoAllocLayout& l, int n_max, dou
   int* perm = reinterpret_cast<int*>(s + l.perm);
   int* off = reinterpret_cast<int*>(s + l.off);
   MtoGArgs<Real> a = mtog_args<Real>(h, 2);
+  a.large = false;  // the matrix-core 6x6 / 8x8 path needs the handle's B, E and Ms workspaces
   a.desc = reinterpret_cast<const SlotDesc*>(s + l.desc);
   a.n_max = n_max;
   a.chunk = 2;  // both spin sectors in one pass over the workspace

@@ -15,9 +15,10 @@ struct MtoGArgs {
   bool fused;             // use the register-fused single-band kernels
   int tw;                 // frequencies per thread in the fused right kernel (0: default)
   int mfma;               // fused right kernel on the matrix cores (complex64, Nc <= 16)
-  // 6x6 and 8x8 clusters: S3 on the matrix cores into the E workspace, then the site transform
-  // and the G epilogue (mtog_large.hip), instead of the register-fused right kernel.
+  // 6x6 and 8x8 clusters: the whole M -> G on the matrix cores with the B, E and Ms workspaces
+  // (mtog_large.hip), instead of the register-fused kernels.
   bool large = false;
+  void* Ms = nullptr;     // [chunk][n_max][n_max] M in sorted order, working precision
   const int* perm;        // [n_slots][n_max]  rows of M, sorted by the vertices' right labels
   const int* off;         // [n_slots][n_orb + 1]
   Cplx<Real>* T;          // [n_slots][n_max][nwE]
@@ -58,8 +59,8 @@ void launch_mtog(const MtoGArgs<Real>& a, hipStream_t st);
 template <typename Real>
 bool launch_mtog_fused(const MtoGArgs<Real>& a, hipStream_t st);
 bool mtog_fused_covers(const Geo& g);
-// Right half of M -> G for single-band 6x6 (NC = 36) and 8x8 (NC = 64) clusters with site labels
-// x + L y, on slots [s0, s0 + ns) of a chunk (needs the E workspace).
+// M -> G for single-band 6x6 (NC = 36) and 8x8 (NC = 64) clusters with site labels x + L y, on
+// slots [s0, s0 + ns) of a chunk (needs the Ms, B and E workspaces).
 template <typename Real, int NC>
 void launch_mtog_large(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st);
 

@@ -531,6 +531,12 @@ void launch_fused_nc(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st) {
   constexpr bool kSep = NC == 4 || NC == 16 || NC == 64;
   const Geo& g = a.g;
   const bool sep = kSep && a.grid_L == grid_side<NC>() && a.kmap;
+  if constexpr (NC == 36 || NC == 64) {
+    if (a.large && a.grid_L == grid_side<NC>() && a.kmap) {
+      launch_mtog_large<Real, NC>(a, s0, ns, st);
+      return;
+    }
+  }
   const dim3 lgrid((g.nwE * a.n_max + 255) / 256, 1, ns);
   if (sep)
     hipLaunchKernelGGL((mtog_left_fused<Real, NC, kSep>), lgrid, dim3(256), 0, st, a.desc, g, a.n_max, s0,
@@ -538,12 +544,6 @@ void launch_fused_nc(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st) {
   else
     hipLaunchKernelGGL((mtog_left_fused<Real, NC, false>), lgrid, dim3(256), 0, st, a.desc, g, a.n_max, s0,
                        a.perm, a.off, a.T, a.dft, nullptr, a.A);
-  if constexpr (NC == 36 || NC == 64) {
-    if (a.large && a.grid_L == grid_side<NC>() && a.kmap) {
-      launch_mtog_large<Real, NC>(a, s0, ns, st);
-      return;
-    }
-  }
   // Matrix-core right kernel (complex64): Nc <= 8, or 16 sites of a 4x4 torus (the dense
   // 16-point DFT epilogue would not fit in registers), with its step table in range.
   if constexpr (std::is_same<Real, float>::value && NC <= 16) {

@@ -1,4 +1,10 @@
-// Right half of M -> G for the large single-band square clusters (6x6 and 8x8), in two kernels:
+// M -> G for the large single-band square clusters (6x6 and 8x8) on the matrix cores. Left half:
+//
+//   mtog_permute_m  M in sorted order (Ms), in the working precision
+//   mtog_s1_mfma    S1: B[jj][r][w] = sum_{ii in site r} Ms[ii][jj] T[w, ii]
+//   mtog_left_site  S2: A[(k, w)][jj] = sum_r exp(+i k.r) B[jj][r][w]
+//
+// Right half, in two kernels:
 //
 //   mtog_s3_mfma  S3 on the matrix cores: for every site r of the column sort
 //                     E[w2][r][p1] = sum_{jj in site r} conj(T[w2, jj]) A[p1, jj]
@@ -186,11 +192,188 @@ __global__ void __launch_bounds__(256) mtog_site_G(Geo g, int slot0, const Cplx<
   }
 }
 
+// Left half, step 0: M of the slot in sorted order, Ms[ii][jj] = M[perm ii, perm jj] (row-major,
+// stride n_max, working precision), with M = diag(exp(V) - 1) N formed here for the CT-AUX
+// hand-off. 64 x 64 tiles through LDS: reads run down M's columns, writes along Ms's rows.
+template <typename Real>
+__global__ void __launch_bounds__(256) mtog_permute_m(const SlotDesc* __restrict__ desc, int n_max, int slot0,
+                                                      const int* __restrict__ perm, Real* __restrict__ Ms) {
+  __shared__ Real tile[64][65];
+  const int slot = slot0 + blockIdx.z;
+  const SlotDesc d = desc[slot];
+  const int ii0 = blockIdx.x * 64, jj0 = blockIdx.y * 64;
+  if (ii0 >= d.n || jj0 >= d.n)
+    return;
+  const int* P = perm + static_cast<size_t>(slot) * n_max;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int a = e & 63, b = e >> 6;  // a: row within the tile (fastest in M's column)
+    const int ii = ii0 + a, jj = jj0 + b;
+    Real v = 0;
+    if (ii < d.n && jj < d.n) {
+      const int i = P[ii];
+      const double m = d.m[i + static_cast<size_t>(d.ld) * P[jj]];
+      v = static_cast<Real>(d.expv ? (d.expv[i] - 1.0) * m : m);
+    }
+    tile[b][a] = v;
+  }
+  __syncthreads();
+  Real* out = Ms + static_cast<size_t>(blockIdx.z) * n_max * n_max;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int b = e & 63, a = e >> 6;  // b: column within the tile (fastest in Ms's row)
+    const int ii = ii0 + a, jj = jj0 + b;
+    if (ii < d.n && jj < d.n)
+      out[jj + static_cast<size_t>(n_max) * ii] = tile[b][a];
+  }
+}
+
+// Left half, step 1 (S1 on the matrix cores): for every site r of the sort,
+//     B[jj][r][w] = sum_{ii in site r} Ms[ii][jj] T[w, ii].
+// A workgroup owns 64 columns jj (16 per wave) and 64 frequencies w (four 16 x 16 tiles per wave)
+// and walks the sites; a step takes four vertices of one site. Lane l supplies Ms[ii0 + l/16][jj]
+// for jj = j0 + l%16 and T[w0 + 16 t + l%16][ii0 + l/16], and holds
+// B[j0 + row(l/16, i)][r][w0 + 16 t + l%16], stored as 16-entry runs of w.
+template <typename Real, int NC>
+__global__ void __launch_bounds__(256) mtog_s1_mfma(const SlotDesc* __restrict__ desc, Geo g, int n_max,
+                                                    int slot0, const int* __restrict__ off,
+                                                    const Cplx<Real>* __restrict__ T,
+                                                    const Real* __restrict__ Ms, Cplx<Real>* __restrict__ B) {
+  using C = Cplx<Real>;
+  using V = typename Mfma16<Real>::acc_t;
+  const int slot = slot0 + blockIdx.z;
+  const int n = desc[slot].n;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j0 = blockIdx.x * 64 + 16 * wave;
+  if (blockIdx.x * 64 >= n)
+    return;
+  const int nwE = g.nwE;
+  const int* O = off + static_cast<size_t>(slot) * (NC + 1);
+  const C* Ts = T + static_cast<size_t>(slot) * nwE * n_max;
+  const Real* Mz = Ms + static_cast<size_t>(blockIdx.z) * n_max * n_max;
+  C* Bz = B + static_cast<size_t>(blockIdx.z) * g.D * n_max;
+  const int kl = lane >> 4;
+  const int jq = j0 + (lane & 15);
+  const bool j_ok = jq < n;
+  const int jc = j_ok ? jq : n - 1;
+  const int w0 = blockIdx.y * 64;
+  int wc[4];
+  bool w_ok[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int w = w0 + 16 * t + (lane & 15);
+    w_ok[t] = w < nwE;
+    wc[t] = w_ok[t] ? w : nwE - 1;
+  }
+  for (int r = 0; r < NC; ++r) {
+    const int b = O[r], e = O[r + 1];
+    V cr[4], ci[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      cr[t] = ci[t] = V{0, 0, 0, 0};
+    for (int i0 = b; i0 < e; i0 += 4) {
+      const int ii = i0 + kl;
+      const bool i_ok = ii < e;
+      const int ic = i_ok ? ii : b;
+      Real m = Mz[jc + static_cast<size_t>(n_max) * ic];
+      if (!(i_ok && j_ok))
+        m = 0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        C tt = Ts[wc[t] + static_cast<size_t>(nwE) * ic];
+        if (!(i_ok && w_ok[t]))
+          tt = C{0, 0};
+        cr[t] = Mfma16<Real>::mma(m, tt.x, cr[t]);
+        ci[t] = Mfma16<Real>::mma(m, tt.y, ci[t]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int jj = j0 + Mfma16<Real>::row(kl, i);
+      if (jj < n)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (w_ok[t])
+            Bz[w0 + 16 * t + (lane & 15) + static_cast<size_t>(nwE) * (r + NC * static_cast<size_t>(jj))] =
+                C{cr[t][i], ci[t][i]};
+    }
+  }
+}
+
+// Left half, step 2: the site transform A[(k, w)][jj] = sum_r exp(+i k.r) B[jj][r][w] for one
+// column jj and 64 frequencies w (one per lane), sites labelled x + L y: pass 1 over x (wave q takes
+// the rows y = q, q + 4, ...), pass 2 over y, then the block of A (NC consecutive momenta per
+// frequency, caller momentum order) is copied out through LDS as one contiguous run.
+template <typename Real, int L>
+__global__ void __launch_bounds__(256) mtog_left_site(const SlotDesc* __restrict__ desc, Geo g, int n_max,
+                                                      int slot0, const Cplx<Real>* __restrict__ B,
+                                                      const int* __restrict__ kmap, Cplx<Real>* __restrict__ A) {
+  using C = Cplx<Real>;
+  constexpr int NC = L * L;
+  __shared__ C F[NC * 64];  // [kx + L y][lane], then [k][lane]
+  __shared__ int Kl[NC];
+  const int slot = slot0 + blockIdx.z;
+  const int jj = blockIdx.y;
+  if (jj >= desc[slot].n)
+    return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwE = g.nwE;
+  const int w0 = blockIdx.x * 64;
+  const int w = w0 + lane;
+  const int wc = w < nwE ? w : nwE - 1;
+  const C* Bj = B + static_cast<size_t>(blockIdx.z) * g.D * n_max + static_cast<size_t>(nwE) * NC * jj + wc;
+  if (threadIdx.x < NC)
+    Kl[threadIdx.x] = kmap[threadIdx.x];
+  for (int y = wave; y < L; y += 4) {
+    C v[L];
+#pragma unroll
+    for (int x = 0; x < L; ++x)
+      v[x] = Bj[static_cast<size_t>(nwE) * (x + L * y)];
+    site_transform_1d<L, 1, Real>(v);
+#pragma unroll
+    for (int kx = 0; kx < L; ++kx)
+      F[(kx + L * y) * 64 + lane] = v[kx];
+  }
+  __syncthreads();
+  constexpr int KX = (L + 3) / 4;  // columns kx per wave
+  C v[KX][L];
+#pragma unroll
+  for (int c = 0; c < KX; ++c) {
+    const int kx = wave + 4 * c;
+    if (kx < L)
+#pragma unroll
+      for (int y = 0; y < L; ++y)
+        v[c][y] = F[(kx + L * y) * 64 + lane];
+  }
+  __syncthreads();  // F is reused for the output block
+#pragma unroll
+  for (int c = 0; c < KX; ++c) {
+    const int kx = wave + 4 * c;
+    if (kx < L) {
+      site_transform_1d<L, 1, Real>(v[c]);
+#pragma unroll
+      for (int ky = 0; ky < L; ++ky)
+        F[Kl[kx + L * ky] * 64 + lane] = v[c][ky];
+    }
+  }
+  __syncthreads();
+  const int nw = min(64, nwE - w0);
+  C* Aj = A + static_cast<size_t>(blockIdx.z) * g.D * n_max + static_cast<size_t>(g.D) * jj +
+          static_cast<size_t>(NC) * w0;
+  for (int e = threadIdx.x; e < NC * nw; e += 256)
+    Aj[e] = F[(e % NC) * 64 + e / NC];
+}
+
 template <typename Real, int NC>
 void launch_mtog_large(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st) {
   constexpr int L = NC == 64 ? 8 : 6;
   static_assert(L * L == NC, "6x6 and 8x8 clusters");
   const Geo& g = a.g;
+  Real* Ms = reinterpret_cast<Real*>(a.Ms);
+  const int nt = (a.n_max + 63) / 64;
+  hipLaunchKernelGGL((mtog_permute_m<Real>), dim3(nt, nt, ns), dim3(256), 0, st, a.desc, a.n_max, s0, a.perm, Ms);
+  hipLaunchKernelGGL((mtog_s1_mfma<Real, NC>), dim3(nt, (g.nwE + 63) / 64, ns), dim3(256), 0, st, a.desc, g,
+                     a.n_max, s0, a.off, a.T, Ms, a.B);
+  hipLaunchKernelGGL((mtog_left_site<Real, L>), dim3((g.nwE + 63) / 64, a.n_max, ns), dim3(256), 0, st, a.desc,
+                     g, a.n_max, s0, a.B, a.kmap, a.A);
   const dim3 g3((g.D + 63) / 64, (g.nwE + 63) / 64, ns);
   hipLaunchKernelGGL((mtog_s3_mfma<Real, NC>), g3, dim3(256), 0, st, g, a.n_max, s0, a.off, a.T, a.A, a.E);
   const dim3 gs((g.D + 63) / 64, g.nwE, ns);

@@ -465,11 +465,19 @@ def test_large_cluster_mtog_equals_fused(name, precision, monkeypatch):
         a, b, c = large.compute_G(meas), reg.compute_G(meas), four.compute_G(meas)
         print(f"{name} {precision} large vs fused {linf(b, a):.3e}, vs four-stage {linf(c, a):.3e}", flush=True)
         assert linf(b, a) < tol and linf(c, a) < tol
-    for acc in (large, reg):
+    # G4 of both paths against the FP64 G4. In complex64 this two-frequency case cancels strongly
+    # in the transverse channel (both paths reach ~1e-6 there), so the bar is that the matrix-core
+    # path is no less accurate than the register path it replaces.
+    ref = g4acc.TpAccumulator(wl.geometry, precision="fp64", max_walkers=2, max_vertices=wl.n_max)
+    for acc in (large, reg, ref):
         acc.resetAccumulation()
         acc.accumulate_batch(ms)
-    for x, y in zip(large.get_G4(), reg.get_G4()):
-        assert linf(y, x) < tol
+    for x, y, r in zip(large.get_G4(), reg.get_G4(), ref.get_G4()):
+        print(f"{name} {precision} G4 large {linf(r, x):.3e}, fused {linf(r, y):.3e} vs FP64", flush=True)
+        if precision == "fp64":
+            assert linf(r, x) < 1e-12 and linf(r, y) < 1e-12
+        else:
+            assert linf(r, x) <= max(1e-6, 1.25 * linf(r, y))
 
 
 @pytest.mark.parametrize("name,dense", [("1band_2x2", False), ("1band_2x2", True),
