@@ -40,6 +40,19 @@ template <typename Real>
 __device__ __forceinline__ bool tile_coords(const TiledArgs<Real>& a, int& w1, int& w2, int& z) {
   const int nw = a.g.nw;
   const int nz = a.g.nv * a.nqg;
+  if (a.sym) {
+    // Transpose symmetry: the nw (nw + 1) / 2 tiles w1 <= w2 in triangular order, w2 slowest.
+    const unsigned L = blockIdx.x / nz;
+    z = blockIdx.x % nz;
+    int r = static_cast<int>((sqrt(8.0 * L + 1.0) - 1.0) * 0.5);
+    while (static_cast<unsigned>((r + 1) * (r + 2) / 2) <= L)
+      ++r;
+    while (static_cast<unsigned>(r * (r + 1) / 2) > L)
+      --r;
+    w2 = r;
+    w1 = static_cast<int>(L) - r * (r + 1) / 2;
+    return w2 < nw;
+  }
   if (a.tile_perm != nullptr) {
     // Schedule 2: logical tiles in the order of the pair table (exchange frequency and momentum
     // group innermost), each XCD taking a contiguous range.
@@ -78,6 +91,12 @@ __device__ __forceinline__ bool tile_coords(const TiledArgs<Real>& a, int& w1, i
 template <typename Real>
 inline unsigned tile_schedule(TiledArgs<Real>& a) {
   const Geo& g = a.g;
+  if (a.sym) {
+    a.tile_perm = nullptr;
+    a.super_tile = a.n_super = 1;
+    a.n_xcd = 0;
+    return static_cast<unsigned>(g.nw) * (g.nw + 1) / 2 * g.nv * a.nqg;
+  }
   if (a.tile_order == 2 && a.tile_perm != nullptr) {
     const unsigned tiles = static_cast<unsigned>(g.nw) * g.nw * g.nv * a.nqg;
     a.super_tile = a.n_super = 1;

@@ -349,6 +349,21 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
       }
     }
   }
+  auto store = [&](int c, size_t idx, double2 v) {
+    if constexpr (RX) {
+      if (a.r1_slot[c] >= 0) {
+        const size_t plane = static_cast<size_t>(nqv) * P * P;
+        const float2 r = static_cast<const float2*>(a.R)[a.r1_slot[c] * plane + idx];
+        v.x += static_cast<double>(r.x);
+        v.y += static_cast<double>(r.y);
+      }
+    }
+    g4_store_t(a.g4[c], idx, v, a.sum_mode);
+  };
+  auto g4_index = [&](int ka, int wa, int kb, int wb, int qvi) {
+    return ka + static_cast<size_t>(nc) *
+                    (wa + static_cast<size_t>(nw) * (kb + static_cast<size_t>(nc) * (wb + static_cast<size_t>(nw) * qvi)));
+  };
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     if (!ok[j])
@@ -357,22 +372,32 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
     for (int qi = 0; qi < QG; ++qi) {
       if (!qok[qi])
         continue;
-      const size_t idx = k1[j] + static_cast<size_t>(nc) *
-                                     (w1 + static_cast<size_t>(nw) *
-                                               (k2[j] + static_cast<size_t>(nc) *
-                                                            (w2 + static_cast<size_t>(nw) * qv[qi])));
+      const size_t idx = g4_index(k1[j], w1, k2[j], w2, qv[qi]);
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+        store(c, idx, tot[j][qi][c]);
+    }
+  }
+  if (a.sym && w1 != w2) {
+    // Mirror tile (w2, w1): G4(k2, w2, k1, w1) = G4(k1, w1, k2, w2), transposed through LDS (the
+    // staging area is free now) so that the stores run along k1 of the mirror tile.
+    double2* tl = reinterpret_cast<double2*>(lds);
+#pragma unroll
+    for (int qi = 0; qi < QG; ++qi) {
+      if (!qok[qi])
+        continue;
 #pragma unroll
       for (int c = 0; c < NCH; ++c) {
-        double2 v = tot[j][qi][c];
-        if constexpr (RX) {
-          if (a.r1_slot[c] >= 0) {
-            const size_t plane = static_cast<size_t>(nqv) * P * P;
-            const float2 r = static_cast<const float2*>(a.R)[a.r1_slot[c] * plane + idx];
-            v.x += static_cast<double>(r.x);
-            v.y += static_cast<double>(r.y);
-          }
-        }
-        g4_store_t(a.g4[c], idx, v, a.sum_mode);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PPT; ++j)
+          if (ok[j])
+            tl[k2[j] + nc * k1[j]] = tot[j][qi][c];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PPT; ++j)
+          if (ok[j])
+            store(c, g4_index(k1[j], w2, k2[j], w1, qv[qi]), tl[k1[j] + nc * k2[j]]);
       }
     }
   }
@@ -516,8 +541,12 @@ bool launch_g4_tiled(TiledArgs<Real> a, hipStream_t st) {
   const dim3 ug((g.nc * g.nw + 255) / 256, g.nq * g.nv, a.n_walkers);
   hipLaunchKernelGGL(uv_kernel<Real>, ug, dim3(256), 0, st, a);
   if constexpr (std::is_same<Real, float>::value) {
-    if (a.mfma && ts.threads == 256 && ppt == 1 && launch_g4_mfma(a, qg, st))
-      return true;
+    if (a.mfma && ts.threads == 256 && ppt == 1) {
+      TiledArgs<float> b = a;
+      b.sym = false;  // the matrix-core tile writes no mirror tiles
+      if (launch_g4_mfma(b, qg, st))
+        return true;
+    }
   }
   auto by_qg = [&](auto tt, auto pp) {
     constexpr int TT = decltype(tt)::value, PP = decltype(pp)::value;

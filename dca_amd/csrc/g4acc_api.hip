@@ -132,6 +132,10 @@ struct g4acc_handle {
   bool large_ok = false;
   int g4_qg = 0;                   // tuning: momenta per G4 workgroup (0: register limit)
   int g4_mfma = 0;                 // tuning: rank-one G4 terms on the matrix cores (FP32 G, Nc <= 16)
+  // tuning: transpose-symmetric G4 tiles (one q = 0, one v = 0, channels among magnetic, charge,
+  // transverse); sym_ok when the geometry allows it.
+  int g4_sym = 0;
+  bool sym_ok = false;
   // tuning: rank-one MAGNETIC / CHARGE terms of a batch as one matrix-core GEMM (g4_rank1.hip),
   // stored in d_R (complex64, one G4-sized plane per channel) and added by the tile kernel.
   int g4_rank1 = 0;
@@ -530,6 +534,7 @@ void run_g4(g4acc_handle* h, int n_walkers) {
     a.tile_perm = h->d_tperm;
     a.sum_mode = h->sum_mode;
     a.mfma = h->g4_mfma;
+    a.sym = h->g4_sym != 0;
     const bool r1 = std::is_same<Real, float>::value && h->g4_rank1 && h->d_R;
     a.R = r1 ? h->d_R : nullptr;
     a.n_r1 = h->n_r1;
@@ -886,6 +891,18 @@ int g4acc_create_model(const g4acc_geometry* gm, const g4acc_options* opt, const
         (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_UV),
                        cb * W2 / 2 * 4 * g.nq * g.nv * g.nc * g.nw, "UV")))
       return bail(rc);
+  }
+  // Transpose symmetry of G4 over ((k1, w1), (k2, w2)): q = 0 (k + q = k for every k), v = 0 and
+  // channels whose terms pair G(p1, p2) with G(p2, p1) only.
+  {
+    bool ok = h->use_tiled && g.nq == 1 && g.nv == 1 && gm->w_ex[0] == 0;
+    for (int k = 0; ok && k < g.nc; ++k)
+      ok = gm->k_add[k + g.nc * gm->k_ex[0]] == k;
+    for (int c : h->channels)
+      ok = ok && (c == G4ACC_PARTICLE_HOLE_MAGNETIC || c == G4ACC_PARTICLE_HOLE_CHARGE ||
+                  c == G4ACC_PARTICLE_HOLE_TRANSVERSE);
+    h->sym_ok = ok && getenv("G4ACC_NO_SYM") == nullptr;
+    h->g4_sym = h->sym_ok ? 1 : 0;
   }
   h->d_g4.assign(h->channels.size(), nullptr);
   for (auto& p : h->d_g4)
@@ -1515,6 +1532,8 @@ int g4acc_set_tuning(g4acc_handle* h, const char* key, int value) {
     h->mtog_mfma = value;
   else if (k == "mtog_large" && (value == 0 || (value == 1 && h->large_ok)))
     h->mtog_large = value;
+  else if (k == "g4_sym" && (value == 0 || (value == 1 && h->sym_ok)))
+    h->g4_sym = value;
   else if (k == "g4_qg" && (value == 0 || value == 1 || value == 2 || value == 4))
     h->g4_qg = value;
   else if (k == "g4_tile_order" && value >= -1 && value <= 2)

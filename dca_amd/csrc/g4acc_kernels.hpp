@@ -118,6 +118,10 @@ struct TiledArgs {
   const int* r1_planes;  // device, n_r1 entries
   int r1_slot[4];
   bool need_y, need_z, has_trans;
+  // Transpose symmetry (one exchange momentum q = 0, one exchange frequency v = 0, channels among
+  // magnetic, charge and transverse): G4 over ((k1, w1), (k2, w2)) is then symmetric, so only the
+  // tiles w1 <= w2 are computed and each also writes its mirror tile (w2, w1).
+  bool sym = false;
 };
 
 template <typename Real>

@@ -480,6 +480,32 @@ def test_large_cluster_mtog_equals_fused(name, precision, monkeypatch):
             assert linf(r, x) <= max(1e-6, 1.25 * linf(r, y))
 
 
+@pytest.mark.parametrize("L,n_freq", [(4, 6), (8, 2)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-6)])
+def test_transpose_symmetric_tiles(oracle_lib, L, n_freq, precision, tol):
+    """With one q = 0 and one v = 0 the magnetic, charge and transverse G4 are symmetric under
+    (k1, w1) <-> (k2, w2); the tile kernel then computes the tiles w1 <= w2 and writes each mirror
+    through LDS. Against the oracle and against the full set of tiles (g4_sym = 0), for single
+    measurements and for batches (several walkers per tile)."""
+    channels = [geo.PARTICLE_HOLE_TRANSVERSE, geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE]
+    wl = synth.tiny_workload(nb=1, L=L, n_freq=n_freq, w_ex=(0,), k_ex=((0.0, 0.0),), channels=channels,
+                             n_range=(10, 50), seed_cfg=640 + L)
+    g = wl.geometry
+    ms = wl.draw(5)
+    ref = oracle_g4(oracle_lib, g, ms)
+    for walkers in (1, 3):
+        sym = g4acc.TpAccumulator(g, precision=precision, max_walkers=walkers, max_vertices=wl.n_max, tuned=False)
+        full = g4acc.TpAccumulator(g, precision=precision, max_walkers=walkers, max_vertices=wl.n_max, tuned=False)
+        full.set_tuning(g4_sym=0)
+        for acc in (sym, full):
+            acc.resetAccumulation()
+            acc.accumulate_batch(ms)
+        for ch, r, x, y in zip(g.channels, ref, sym.get_G4(), full.get_G4()):
+            print(f"L={L} {precision} walkers={walkers} {geo.CHANNEL_NAMES[ch]}: symmetric tiles "
+                  f"{linf(r, x):.3e}, all tiles {linf(r, y):.3e}, between {linf(y, x):.3e}", flush=True)
+            assert linf(r, x) < tol and linf(y, x) < (1e-13 if precision == "fp64" else 1e-6)
+
+
 @pytest.mark.parametrize("name,dense", [("1band_2x2", False), ("1band_2x2", True),
                                         ("1band_4x4_4q", False), ("1band_8site", False)])
 def test_mfma_right_mtog_matches_fp64(name, dense, monkeypatch):
