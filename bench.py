@@ -88,21 +88,23 @@ def host_info():
     return {"nproc": cores, "cpu_model": model, "mem_available_gb": mem_gb}
 
 
-def measured_traffic(cfg, walkers):
-    """HBM bytes per launch of the G4 stage from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the same workload and batch size, or None."""
-    path = os.path.join(ROOT, "profiles", f"g4_update_cfg{cfg}_hbm_traffic.txt")
-    if not os.path.exists(path):
-        return None, None
-    traffic, w = None, None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            traffic = int(line.split(":")[1])
-        if line.startswith("walkers_per_launch:"):
-            w = int(line.split(":")[1])
-    if traffic is None or w != walkers:
-        return None, None
-    return traffic, os.path.relpath(path, ROOT)
+def measured_traffic(cfg, walkers, key="traffic_bytes_per_launch"):
+    """HBM bytes of one batch's G4 stage (key traffic_bytes_per_launch) or M -> G stage (key
+    mtog_traffic_bytes_per_batch) from the committed rocprofv3 --pmc summary (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated) of the same workload and batch size, or None."""
+    for name in (f"mtog_g4_cfg{cfg}_hbm_traffic.txt", f"g4_update_cfg{cfg}_hbm_traffic.txt"):
+        path = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(path):
+            continue
+        traffic, w = None, None
+        for line in open(path):
+            if line.startswith(key + ":"):
+                traffic = int(line.split(":")[1])
+            if line.startswith("walkers_per_launch:"):
+                w = int(line.split(":")[1])
+        if traffic is not None and w == walkers:
+            return traffic, os.path.relpath(path, ROOT)
+    return None, None
 
 
 def run_batches(acc, batches, steps):
@@ -246,29 +248,24 @@ def roofline(cfg, ctx):
     mtog_gbs = mtog_bytes / (mtog_ms * 1e-3) / 1e9
     flops = sum(mtog_flops(g, n) for m in pool for n in m.n)
     mtog_tf = flops / (mtog_ms * 1e-3) / 1e12
+    g4_traffic, g4_src = measured_traffic(cfg, W)
+    mtog_traffic, mtog_src = measured_traffic(cfg, W, "mtog_traffic_bytes_per_batch")
+    hbm_frac, fp_frac = mtog_gbs / HBM_PEAK_GBS, mtog_tf / FP32_PEAK_TFLOPS
     if g4_ms >= mtog_ms:
-        traffic, src = measured_traffic(cfg, W)
         return {"bound": "hbm", "achieved": round(g4_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(g4_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "frac": round(g4_gbs / HBM_PEAK_GBS, 4), "traffic": g4_traffic,
                 "kernel": "G4 update stage (uv_kernel + g4_tiled_kernel per batch)",
                 "dominant_stage": "g4", "algorithmic_bytes_per_launch": g4_bytes,
-                "traffic_source": src,
+                "traffic_source": g4_src,
                 "other_stage": {"stage": "mtog", "ms": round(mtog_ms, 4), "achieved_tflops": round(mtog_tf, 2),
-                                "frac_fp32": round(mtog_tf / FP32_PEAK_TFLOPS, 4),
-                                "achieved_gbs": round(mtog_gbs, 1)}}
-    hbm_frac, fp_frac = mtog_gbs / HBM_PEAK_GBS, mtog_tf / FP32_PEAK_TFLOPS
-    traffic_path = os.path.join(ROOT, "profiles", f"mtog_cfg{cfg}_hbm_traffic.txt")
-    traffic = None
-    if os.path.exists(traffic_path):
-        for line in open(traffic_path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                traffic = int(line.split(":")[1])
+                                "frac_fp32": round(fp_frac, 4), "achieved_gbs": round(mtog_gbs, 1),
+                                "frac_hbm": round(hbm_frac, 4), "traffic": mtog_traffic}}
     common = {"kernel": "M -> G stage (sort, phase, left and right M -> G kernels per batch)",
-              "dominant_stage": "mtog", "traffic": traffic,
-              "traffic_source": os.path.relpath(traffic_path, ROOT) if traffic else None,
+              "dominant_stage": "mtog", "traffic": mtog_traffic, "traffic_source": mtog_src,
               "algorithmic_flops_per_launch": flops, "algorithmic_bytes_per_launch": mtog_bytes,
               "other_stage": {"stage": "g4", "ms": round(g4_ms, 4), "achieved_gbs": round(g4_gbs, 1),
-                              "frac_hbm": round(g4_gbs / HBM_PEAK_GBS, 4)}}
+                              "frac_hbm": round(g4_gbs / HBM_PEAK_GBS, 4), "traffic": g4_traffic,
+                              "algorithmic_bytes_per_launch": g4_bytes}}
     if fp_frac >= hbm_frac:
         return {"bound": "mfma", "achieved": round(mtog_tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(fp_frac, 4), "hbm_frac": round(hbm_frac, 4), **common}
