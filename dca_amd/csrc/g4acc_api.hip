@@ -126,10 +126,11 @@ struct g4acc_handle {
   bool use_graph = true;
   int mtog_tw = 0;                 // tuning: frequencies per thread of the fused M -> G kernel
   int mtog_mfma = 0;               // tuning: fused right M -> G kernel on the matrix cores
-  // 6x6 / 8x8 square clusters: matrix-core S3 into the E workspace plus the site transform
-  // kernel (mtog_large.hip); large_ok when E was sized for it.
+  // 6x6 / 8x8 square clusters, the matrix-core M -> G of mtog_large.hip (large_ok): 1 = S3 into
+  // the E workspace plus the site transform kernel (large_e: E was allocated), 2 = FP32 only, S3,
+  // site transform and epilogue in one kernel with E in LDS (the FP32 default).
   int mtog_large = 0;
-  bool large_ok = false;
+  bool large_ok = false, large_e = false, right_fused_ok = false;
   int g4_qg = 0;                   // tuning: momenta per G4 workgroup (0: register limit)
   int g4_mfma = 0;                 // tuning: rank-one G4 terms on the matrix cores (FP32 G, Nc <= 16)
   // tuning: transpose-symmetric G4 tiles (one q = 0, one v = 0, channels among magnetic, charge,
@@ -476,6 +477,7 @@ MtoGArgs<Real> mtog_args(g4acc_handle* h, int n_slots) {
   a.tw = h->mtog_tw;
   a.mfma = h->mtog_mfma;
   a.large = h->mtog_large != 0;
+  a.right_fused = h->mtog_large == 2;
   a.Ms = h->d_Ms;
   a.perm = h->d_perm;
   a.off = h->d_off;
@@ -829,11 +831,16 @@ int g4acc_create_model(const g4acc_geometry* gm, const g4acc_options* opt, const
   if (getenv("G4ACC_DENSE_DFT") == nullptr)
     h->grid_L = detect_grid(g.nc, gm->dft, xy, kxy_to_k);
   h->large_ok = h->fused_mtog && (g.nc == 36 || g.nc == 64) && h->grid_L * h->grid_L == g.nc;
-  h->mtog_large = h->large_ok ? 1 : 0;
-  // Per slot of a chunk: A (D x N); the matrix-core 6x6 / 8x8 path adds B (D x N), E (D x D) and
-  // the sorted M (N x N real); the four-stage path B and E.
+  // The FP32 right kernel with E in registers and LDS indexes a slot's A and G with 32-bit
+  // offsets. E is allocated for FP64, where that kernel does not exist, for larger shapes, or on
+  // request.
+  h->right_fused_ok = h->large_ok && cb == 8 && D * D < (size_t(1) << 32) && D * NM < (size_t(1) << 32);
+  h->large_e = h->large_ok && (!h->right_fused_ok || getenv("G4ACC_MTOG_E") != nullptr);
+  h->mtog_large = !h->large_ok ? 0 : h->right_fused_ok ? 2 : 1;
+  // Per slot of a chunk: A (D x N); the matrix-core 6x6 / 8x8 path adds B (D x N), the sorted M
+  // (N x N real) and E (D x D) when allocated; the four-stage path B and E.
   const size_t per_slot = !h->fused_mtog ? cb * (2 * D * NM + D * D)
-                          : h->large_ok  ? cb * (2 * D * NM + D * D) + cb / 2 * NM * NM
+                          : h->large_ok  ? cb * (2 * D * NM + (h->large_e ? D * D : 0)) + cb / 2 * NM * NM
                                          : cb * D * NM;
   h->chunk = static_cast<int>(std::max<size_t>(1, std::min<size_t>(W2, workspace_budget_bytes() / per_slot)));
   const size_t CH = h->chunk;
@@ -852,7 +859,7 @@ int g4acc_create_model(const g4acc_geometry* gm, const g4acc_options* opt, const
                      "sorted M")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_A), cb * CH * NM * D, "A")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_E),
-                     h->fused_mtog && !h->large_ok ? 1 : cb * CH * D * D, "E")) ||
+                     h->fused_mtog && !h->large_e ? 1 : cb * CH * D * D, "E")) ||
       (rc = h->alloc(reinterpret_cast<unsigned char**>(&h->d_G), cb * W2 * D * D, "G")) ||
       (rc = h->alloc(&h->d_desc, W2, "desc")))
     return bail(rc);
@@ -1530,7 +1537,8 @@ int g4acc_set_tuning(g4acc_handle* h, const char* key, int value) {
     h->mtog_tw = value;
   else if (k == "mtog_mfma" && (value == 0 || value == 1))
     h->mtog_mfma = value;
-  else if (k == "mtog_large" && (value == 0 || (value == 1 && h->large_ok)))
+  else if (k == "mtog_large" &&
+           (value == 0 || (value == 1 && h->large_e) || (value == 2 && h->right_fused_ok)))
     h->mtog_large = value;
   else if (k == "g4_sym" && (value == 0 || (value == 1 && h->sym_ok)))
     h->g4_sym = value;

@@ -18,6 +18,9 @@ struct MtoGArgs {
   // 6x6 and 8x8 clusters: the whole M -> G on the matrix cores with the B, E and Ms workspaces
   // (mtog_large.hip), instead of the register-fused kernels.
   bool large = false;
+  // FP32 only: S3, the right site transform and the epilogue in one kernel, E kept in registers
+  // and LDS (no E workspace).
+  bool right_fused = false;
   void* Ms = nullptr;     // [chunk][n_max][n_max] M in sorted order, working precision
   const int* perm;        // [n_slots][n_max]  rows of M, sorted by the vertices' right labels
   const int* off;         // [n_slots][n_orb + 1]

@@ -4,7 +4,7 @@
 //   mtog_s1_mfma    S1: B[jj][r][w] = sum_{ii in site r} Ms[ii][jj] T[w, ii]
 //   mtog_left_site  S2: A[(k, w)][jj] = sum_r exp(+i k.r) B[jj][r][w]
 //
-// Right half, in two kernels:
+// Right half, in two kernels (FP64, or FP32 with E as a workspace):
 //
 //   mtog_s3_mfma  S3 on the matrix cores: for every site r of the column sort
 //                     E[w2][r][p1] = sum_{jj in site r} conj(T[w2, jj]) A[p1, jj]
@@ -18,7 +18,10 @@
 // and thread; at Nc = 64 that leaves one or three waves per SIMD and no room for matrix-core tiles.
 // Written out, E costs one D x D write and read per slot, and both kernels run at full occupancy:
 // S3 as 16 x 16 x 4 matrix-core steps of four vertices, the site transform with eight (or six)
-// values per thread.
+// values per thread. In FP32 the default is one kernel, mtog_right_block, that keeps E of a
+// 16 x 16 (p1, w2) block in registers and LDS instead.
+#include <type_traits>
+
 #include "g4acc_common.hpp"
 #include "g4acc_kernels.hpp"
 #include "site_fft.hpp"
@@ -188,6 +191,230 @@ __global__ void __launch_bounds__(256) mtog_site_G(Geo g, int slot0, const Cplx<
         val.y = fma(ga.y, g.beta, val.y);
       }
       Gs[static_cast<size_t>(D) * p2] = from_d<C>(val);
+    }
+  }
+}
+
+// The right half in one kernel (FP32): S3, the site transform and the G epilogue of a 16 x 16 block
+// of (p1, w2) pairs. Wave y owns the row of sites x + L y, x = 0..L-1, whose vertices are one
+// contiguous range of the column sort. It walks that range as one stream of steps of four
+// vertices (a step never spans two sites): lane l supplies conj-T[w0 + l%16][jj0 + l/16] and
+// A[p0 + l%16][jj0 + l/16] to a 16 x 16 x 4 matrix-core product, with the loads PF steps ahead of
+// the products and consecutive steps alternating between two accumulator sets. At the end of
+// site x its tile goes to the registers of x (lane l holds the pairs (w2 local, p1 local) =
+// (row(l/16, i), l%16), i = 0..3) through a branch on x, which is uniform, and after the row the
+// transform over x runs in registers. The transform over y needs every row: two momenta kx at a
+// time go through LDS (2 x L x 256 values, 32 KiB at L = 8; with the block's G0 values 48 KiB, so
+// two workgroups share a CU and one's epilogue overlaps the other's products), and each thread
+// transforms (pair, kx) tasks over y and writes G in runs of 16 consecutive p1 (128 bytes) after
+// the FP64 epilogue -G0(p1) M G0(p2) / Nc + beta G0(p1) delta.
+// Workgroups that share a p1 block (and so read the same 16 rows of A) are given consecutive
+// logical ids on one XCD when the block count allows it, so A is read from HBM about once.
+// Offsets within a slot's A, T and G fit in 32 bits (the handle selects this kernel only then),
+// which keeps the per-step address arithmetic to one multiply-add per load.
+template <int L>
+__global__ void __launch_bounds__(64 * L, 4) mtog_right_block(Geo g, int n_max, int slot0, int n_wb, int xcd_map,
+                                                            const int* __restrict__ off,
+                                                            const Cplx<float>* __restrict__ T,
+                                                            const Cplx<float>* __restrict__ A,
+                                                            const double2* __restrict__ bphase,
+                                                            const double2* __restrict__ G0,
+                                                            const int* __restrict__ kmap,
+                                                            Cplx<float>* __restrict__ G) {
+  using C = Cplx<float>;
+  using V = Mfma16<float>::acc_t;
+  constexpr int NC = L * L, NT = 64 * L, KR = 2;
+  constexpr int PF = 4;  // steps in flight
+  __shared__ C X[KR * L * 256];  // [kx - kx0][y][w2 local * 16 + p1 local]
+  __shared__ double2 Gb[16 * NC];  // G0(k2, w2) of the block's 16 frequencies w2, k2 fastest
+  __shared__ double2 Ga[16];       // G0(p1) of the block's 16 rows p1
+  __shared__ int Kl[NC];
+  __shared__ int Ol[NC + 1];  // the slot's site offsets
+  const int slot = slot0 + blockIdx.z;
+  const int s = slot & 1;
+  const int lane = threadIdx.x & 63;
+  const int y = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int D = g.D, nwE = g.nwE;
+  unsigned lid = blockIdx.x;
+  if (xcd_map)
+    lid = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+  const int p0 = static_cast<int>(lid / n_wb) * 16, w0 = static_cast<int>(lid % n_wb) * 16;
+  if (threadIdx.x < NC)
+    Kl[threadIdx.x] = kmap[threadIdx.x];
+  // Site offsets through LDS: reading them in the step loop then waits on LDS only, not on the
+  // vector loads in flight.
+  if (threadIdx.x <= NC)
+    Ol[threadIdx.x] = off[static_cast<size_t>(slot) * (NC + 1) + threadIdx.x];
+  for (int t = threadIdx.x; t < 16 * NC; t += NT) {
+    const int w2 = w0 + t / NC;
+    Gb[t] = w2 < nwE ? G0[s + 2 * (static_cast<size_t>(t % NC) + static_cast<size_t>(NC) * w2)] : double2{0, 0};
+  }
+  if (threadIdx.x < 16)
+    Ga[threadIdx.x] = p0 + threadIdx.x < D ? G0[s + 2 * static_cast<size_t>(p0 + threadIdx.x)] : double2{0, 0};
+  __syncthreads();
+
+  const int jl = lane >> 4;
+  const int pl = p0 + (lane & 15), wl = w0 + (lane & 15);
+  // Per-slot bases (uniform) and this lane's element offsets within them.
+  const C* Ab = A + static_cast<size_t>(blockIdx.z) * D * n_max;
+  const C* Tb = T + static_cast<size_t>(slot) * nwE * n_max;
+  const unsigned a_lane = static_cast<unsigned>(pl < D ? pl : D - 1);
+  const unsigned t_lane = static_cast<unsigned>(wl < nwE ? wl : nwE - 1);
+  const int* Orow = Ol + L * y;
+  // Sites of the row with vertices, as a bit mask (uniform).
+  unsigned full = 0;
+#pragma unroll
+  for (int x = 0; x < L; ++x)
+    if (Orow[x] < Orow[x + 1])
+      full |= 1u << x;
+  full = __builtin_amdgcn_readfirstlane(full);
+
+  // Step cursor (uniform across the wave): site cx (L: done), its last vertex cl, next step's
+  // first vertex cj. Advancing has no loop.
+  int cx = L, cj = 0, cl = 0;
+  auto open_site = [&](int from) {
+    const unsigned rest = from < L ? full >> from : 0u;
+    cx = rest ? from + __builtin_ctz(rest) : L;
+    if (cx < L) {
+      cj = __builtin_amdgcn_readfirstlane(Orow[cx]);
+      cl = __builtin_amdgcn_readfirstlane(Orow[cx + 1]) - 1;
+    }
+  };
+  open_site(0);
+  C xs[PF], ts[PF];
+  int rs[PF];  // site of the step in the ring position (L: none)
+  float ms[PF];  // 1 for vertices of the site, 0 past its end
+  auto fetch = [&](int k) {
+    rs[k] = cx;
+    const int jj = cj + jl;
+    // Past the site's last vertex the lane reads that vertex again (finite data) and is masked.
+    ms[k] = cx < L && jj <= cl ? 1.0f : 0.0f;
+    const unsigned jc = static_cast<unsigned>(jj <= cl ? jj : cl);
+    xs[k] = Ab[a_lane + jc * static_cast<unsigned>(D)];
+    ts[k] = Tb[t_lane + jc * static_cast<unsigned>(nwE)];
+    if (cx < L) {
+      cj += 4;
+      if (cj > cl)
+        open_site(cx + 1);
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < PF; ++k)
+    fetch(k);
+
+  C e[4][L];  // [pair i][site x], then [pair i][kx]
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int x = 0; x < L; ++x)
+      e[i][x] = C{0, 0};
+  V cr0{0, 0, 0, 0}, ci0{0, 0, 0, 0}, cr1{0, 0, 0, 0}, ci1{0, 0, 0, 0};
+  int acc_x = rs[0];
+  auto flush = [&]() {
+    C t[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      t[i] = C{cr0[i] + cr1[i], ci0[i] + ci1[i]};
+    switch (acc_x) {
+#define G4ACC_SITE_CASE(X)          \
+  case X:                           \
+    if constexpr (X < L)            \
+      for (int i = 0; i < 4; ++i)   \
+        e[i][X < L ? X : 0] = t[i]; \
+    break;
+      G4ACC_SITE_CASE(0)
+      G4ACC_SITE_CASE(1)
+      G4ACC_SITE_CASE(2)
+      G4ACC_SITE_CASE(3)
+      G4ACC_SITE_CASE(4)
+      G4ACC_SITE_CASE(5)
+      G4ACC_SITE_CASE(6)
+      G4ACC_SITE_CASE(7)
+#undef G4ACC_SITE_CASE
+      default:
+        break;
+    }
+    cr0 = ci0 = cr1 = ci1 = V{0, 0, 0, 0};
+  };
+  static_assert(L <= 8, "site cases above");
+  // Every ring position issues its two loads on every pass, also past the stream's end (dummy
+  // steps on a valid address), so the loads in flight are the same on every path through the
+  // loop and the wait before a step covers only that step's loads. Steps past the end skip the
+  // products.
+  while (rs[0] < L) {
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      if (rs[k] < L) {
+        if (rs[k] != acc_x) {
+          flush();
+          acc_x = rs[k];
+        }
+        // Vertices past the site's end: A scaled by 0, not selected in place, which would make
+        // the compiler wait for every load in flight.
+        const C xa{xs[k].x * ms[k], xs[k].y * ms[k]}, ta = ts[k];
+        // E += conj(T) A = (Tx Ax + Ty Ay) + i (Tx Ay - Ty Ax)
+        if (k % 2 == 0) {
+          cr0 = Mfma16<float>::mma(ta.x, xa.x, cr0);
+          ci0 = Mfma16<float>::mma(ta.x, xa.y, ci0);
+          cr0 = Mfma16<float>::mma(ta.y, xa.y, cr0);
+          ci0 = Mfma16<float>::mma(-ta.y, xa.x, ci0);
+        }
+        else {
+          cr1 = Mfma16<float>::mma(ta.x, xa.x, cr1);
+          ci1 = Mfma16<float>::mma(ta.x, xa.y, ci1);
+          cr1 = Mfma16<float>::mma(ta.y, xa.y, cr1);
+          ci1 = Mfma16<float>::mma(-ta.y, xa.x, ci1);
+        }
+      }
+      fetch(k);
+    }
+  }
+  if (acc_x < L)
+    flush();
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    site_transform_1d<L, -1, float>(e[i]);
+
+  C* Gs = G + static_cast<size_t>(slot) * D * D;
+  const double inv_nc = 1.0 / NC;
+#pragma unroll
+  for (int kx0 = 0; kx0 < L; kx0 += KR) {
+    if (kx0 > 0)
+      __syncthreads();  // the previous momenta have been read
+#pragma unroll
+    for (int kk = 0; kk < KR; ++kk)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        X[(kk * L + y) * 256 + Mfma16<float>::row(jl, i) * 16 + (lane & 15)] = e[i][kx0 + kk];
+    __syncthreads();
+    for (int t = threadIdx.x; t < KR * 256; t += NT) {
+      const int pr = t & 255, kk = t >> 8;
+      const int p1 = p0 + (pr & 15), w2 = w0 + (pr >> 4);
+      if (p1 >= D || w2 >= nwE)
+        continue;
+      const int kx = kx0 + kk;
+      const double2 ga = Ga[pr & 15];                       // G0(k1, w1), p1 = k1 + Nc w1
+      const double2 gneg{-ga.x * inv_nc, -ga.y * inv_nc};  // -G0(p1) / Nc
+      C v[L];
+#pragma unroll
+      for (int yy = 0; yy < L; ++yy)
+        v[yy] = X[(kk * L + yy) * 256 + pr];
+      site_transform_1d<L, -1, float>(v);
+      const int k1 = p1 % NC;
+#pragma unroll
+      for (int ky = 0; ky < L; ++ky) {
+        const int k2 = Kl[kx + L * ky];
+        const unsigned p2 = static_cast<unsigned>(k2 + NC * w2);
+        double2 m = to_d(v[ky]);
+        if (bphase)
+          m = cmulc(cmul(m, bphase[k1]), bphase[k2]);
+        double2 val = cmul(cmul(gneg, m), Gb[k2 + NC * (pr >> 4)]);
+        if (static_cast<unsigned>(p1) == p2) {
+          val.x = fma(ga.x, g.beta, val.x);
+          val.y = fma(ga.y, g.beta, val.y);
+        }
+        Gs[static_cast<unsigned>(p1) + p2 * static_cast<unsigned>(D)] = from_d<C>(val);
+      }
     }
   }
 }
@@ -374,6 +601,16 @@ void launch_mtog_large(const MtoGArgs<Real>& a, int s0, int ns, hipStream_t st) 
                      a.n_max, s0, a.off, a.T, Ms, a.B);
   hipLaunchKernelGGL((mtog_left_site<Real, L>), dim3((g.nwE + 63) / 64, a.n_max, ns), dim3(256), 0, st, a.desc,
                      g, a.n_max, s0, a.B, a.kmap, a.A);
+  if constexpr (std::is_same<Real, float>::value) {
+    // Set by the handle only where a slot's A, T and G fit 32-bit offsets.
+    if (a.right_fused) {
+      const int n_wb = (g.nwE + 15) / 16;
+      const unsigned nblk = static_cast<unsigned>((g.D + 15) / 16) * n_wb;
+      hipLaunchKernelGGL((mtog_right_block<L>), dim3(nblk, 1, ns), dim3(64 * L), 0, st, g, a.n_max, s0,
+                         n_wb, nblk % 8 == 0 ? 1 : 0, a.off, a.T, a.A, a.bphase, a.G0, a.kmap, a.G);
+      return;
+    }
+  }
   const dim3 g3((g.D + 63) / 64, (g.nwE + 63) / 64, ns);
   hipLaunchKernelGGL((mtog_s3_mfma<Real, NC>), g3, dim3(256), 0, st, g, a.n_max, s0, a.off, a.T, a.A, a.E);
   const dim3 gs((g.D + 63) / 64, g.nwE, ns);

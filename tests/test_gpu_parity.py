@@ -449,35 +449,43 @@ def test_fused_equals_unfused_mtog(name, precision, monkeypatch):
 @pytest.mark.parametrize("name", ["1band_6x6", "1band_8x8"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_large_cluster_mtog_equals_fused(name, precision, monkeypatch):
-    """6x6 and 8x8: the matrix-core S3 plus site-transform kernels (default) against the
-    register-fused right kernel (mtog_large = 0) and the four-stage kernels, over batches of
-    two walkers (four slots: the chunked E workspace) with uneven vertex counts."""
+    """6x6 and 8x8: the matrix-core kernels (default: in FP32 S3, site transform and epilogue in
+    one kernel; in FP64 S3 into the E workspace plus the site-transform kernel) against the
+    register-fused right kernel (mtog_large = 0), the four-stage kernels and, in FP32, the E
+    workspace path (mtog_large = 1), over batches of two walkers (four slots: the chunked
+    workspaces) with uneven vertex counts."""
     wl = case(name)
     ms = wl.draw(2)
     large = g4acc.TpAccumulator(wl.geometry, precision=precision, max_walkers=2, max_vertices=wl.n_max)
     reg = g4acc.TpAccumulator(wl.geometry, precision=precision, max_walkers=2, max_vertices=wl.n_max,
                               tuned=False)
     reg.set_tuning(mtog_large=0)
+    monkeypatch.setenv("G4ACC_MTOG_E", "1")
+    with_e = g4acc.TpAccumulator(wl.geometry, precision=precision, max_walkers=2, max_vertices=wl.n_max,
+                                 tuned=False)
+    with_e.set_tuning(mtog_large=1)
     monkeypatch.setenv("G4ACC_UNFUSED_MTOG", "1")
     four = g4acc.TpAccumulator(wl.geometry, precision=precision, max_vertices=wl.n_max)
     tol = 1e-14 if precision == "fp64" else 1e-6
     for meas in ms:
-        a, b, c = large.compute_G(meas), reg.compute_G(meas), four.compute_G(meas)
-        print(f"{name} {precision} large vs fused {linf(b, a):.3e}, vs four-stage {linf(c, a):.3e}", flush=True)
-        assert linf(b, a) < tol and linf(c, a) < tol
+        a, b, c, d = large.compute_G(meas), reg.compute_G(meas), four.compute_G(meas), with_e.compute_G(meas)
+        print(f"{name} {precision} large vs fused {linf(b, a):.3e}, vs four-stage {linf(c, a):.3e}, "
+              f"vs E workspace {linf(d, a):.3e}", flush=True)
+        assert linf(b, a) < tol and linf(c, a) < tol and linf(d, a) < tol
     # G4 of both paths against the FP64 G4. In complex64 this two-frequency case cancels strongly
     # in the transverse channel (both paths reach ~1e-6 there), so the bar is that the matrix-core
     # path is no less accurate than the register path it replaces.
     ref = g4acc.TpAccumulator(wl.geometry, precision="fp64", max_walkers=2, max_vertices=wl.n_max)
-    for acc in (large, reg, ref):
+    for acc in (large, reg, with_e, ref):
         acc.resetAccumulation()
         acc.accumulate_batch(ms)
-    for x, y, r in zip(large.get_G4(), reg.get_G4(), ref.get_G4()):
-        print(f"{name} {precision} G4 large {linf(r, x):.3e}, fused {linf(r, y):.3e} vs FP64", flush=True)
+    for x, y, z, r in zip(large.get_G4(), reg.get_G4(), with_e.get_G4(), ref.get_G4()):
+        print(f"{name} {precision} G4 large {linf(r, x):.3e}, fused {linf(r, y):.3e}, "
+              f"E workspace {linf(r, z):.3e} vs FP64", flush=True)
         if precision == "fp64":
-            assert linf(r, x) < 1e-12 and linf(r, y) < 1e-12
+            assert linf(r, x) < 1e-12 and linf(r, y) < 1e-12 and linf(r, z) < 1e-12
         else:
-            assert linf(r, x) <= max(1e-6, 1.25 * linf(r, y))
+            assert linf(r, x) <= max(1e-6, 1.25 * linf(r, y)) and linf(r, z) <= max(1e-6, 1.25 * linf(r, y))
 
 
 @pytest.mark.parametrize("L,n_freq", [(4, 6), (8, 2)])
