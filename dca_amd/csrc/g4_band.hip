@@ -13,8 +13,8 @@
 //   PP_UP_DOWN  += a sum_s X_s(b1,b3) Z_{1-s}(b2,b4)
 //   TRANSVERSE  -= a sum_s X_s(b2,b4) Y_{1-s}(b3,b1)
 // with X(b,b') = X(b,b'; k1,k2), Y(b,b') = Y(b,b'; k2+q, k1+q), Z(b,b') = Z(b,b'; q-k1, q-k2).
-// The factors that do not depend on b4 are read once per walker. With complex64 G the terms are
-// summed in FP32 for 8 walkers, then added to FP64 totals; G4 is read and written once per launch.
+// The factors that do not depend on b4 are read once per walker. Each walker's terms are formed in
+// the G precision and added to FP64 totals; G4 is read and written once per launch.
 #include "g4_tile_common.hpp"
 
 namespace g4acc {
@@ -32,8 +32,6 @@ __global__ void __launch_bounds__(kBandThreads) g4_band_kernel(TiledArgs<Real> a
   constexpr bool kChg = (MASK & (1 << PH_CHARGE)) != 0;
   constexpr bool kPP = (MASK & (1 << PP_UP_DOWN)) != 0;
   constexpr bool kTr = (MASK & (1 << PH_TRANSVERSE)) != 0;
-  constexpr bool kChunk = sizeof(Real) == 4;
-  constexpr int FLUSH = 8;
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const Geo& g = a.g;
@@ -41,8 +39,12 @@ __global__ void __launch_bounds__(kBandThreads) g4_band_kernel(TiledArgs<Real> a
   const int R = NB * nc, ldb = R + 1;  // block rows b + NB k, padded column stride
   const int bsz = R * ldb, nbk = NB * NB * nc;
   const size_t D = g.D, DD = D * D;
-  // Tile: b1 fastest (the workgroups of one tile stage the same blocks), then q, v, w1, w2.
+  // Tile: b1 fastest (the workgroups of one tile stage the same blocks), then q, v, w1, w2. The
+  // workgroups are dealt to the 8 XCDs round-robin by id; each XCD gets a contiguous range of
+  // tiles, so the workgroups that stage the same blocks share its L2.
   int t = blockIdx.x;
+  if (gridDim.x % 8 == 0)
+    t = static_cast<int>((blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8);
   const int b1 = t % NB;
   t /= NB;
   const int iq = t % g.nq;
@@ -106,136 +108,126 @@ __global__ void __launch_bounds__(kBandThreads) g4_band_kernel(TiledArgs<Real> a
   auto yi = [&](int bA, int bB) { return (bB + NB * kq1) * ldb + bA + NB * kq2; };  // Y(bA,bB;k2+q,k1+q)
   auto zi = [&](int bA, int bB) { return (bB + NB * qk2) * ldb + bA + NB * qk1; };  // Z(bA,bB;q-k1,q-k2)
 
-  C xm[NB], xc[NB], pz[NB], tr[NB], dm[NB], dc[NB];
+  // FP64 totals per entry b4 and term. Each walker's terms are formed in the G precision (both
+  // spins summed) and added here, so no partial sums are kept across walkers.
   double2 tot[NB][4];
 #pragma unroll
-  for (int b4 = 0; b4 < NB; ++b4) {
-    xm[b4] = xc[b4] = pz[b4] = tr[b4] = dm[b4] = dc[b4] = C{0, 0};
+  for (int b4 = 0; b4 < NB; ++b4)
 #pragma unroll
     for (int c = 0; c < 4; ++c)
       tot[b4][c] = double2{0, 0};
-  }
-  // Adds one set of term sums to the FP64 totals of entry b4 (only the channels of MASK).
-  auto add_terms = [&](int b4, const C& vdm, const C& vxm, const C& vdc, const C& vxc, const C& vpz,
-                       const C& vtr) {
-    auto add = [](double2& t, Real x, Real y) {
-      t.x += static_cast<double>(x);
-      t.y += static_cast<double>(y);
-    };
-    if (kMag)
-      add(tot[b4][0], vdm.x - vxm.x, vdm.y - vxm.y);
-    if (kChg)
-      add(tot[b4][1], vdc.x - vxc.x, vdc.y - vxc.y);
-    if (kPP)
-      add(tot[b4][2], vpz.x, vpz.y);
-    if (kTr)
-      add(tot[b4][3], -vtr.x, -vtr.y);
-  };
-  auto flush = [&]() {
-#pragma unroll
-    for (int b4 = 0; b4 < NB; ++b4) {
-      add_terms(b4, dm[b4], xm[b4], dc[b4], xc[b4], pz[b4], tr[b4]);
-      xm[b4] = xc[b4] = pz[b4] = tr[b4] = dm[b4] = dc[b4] = C{0, 0};
-    }
+  auto add = [](double2& t, Real x, Real y) {
+    t.x += static_cast<double>(x);
+    t.y += static_cast<double>(y);
   };
 
-  C pre[kBandPre];
-  auto prefetch = [&](int m) {
+  // Staged values of walker m + DEPTH are loaded while walker m is consumed (DEPTH register sets:
+  // two for complex64 G, one for complex128, whose second set would not fit in registers).
+  constexpr int DEPTH = sizeof(Real) == 4 ? 2 : 1;
+  C pre[DEPTH][kBandPre];
+  auto prefetch = [&](int m, C (&pr)[kBandPre]) {
     const C* H = a.G + (2 * static_cast<size_t>(m)) * DD;
 #pragma unroll
     for (int j = 0; j < kBandPre; ++j)
       if (src[j] >= 0)
-        pre[j] = H[src[j]];
+        pr[j] = H[src[j]];
   };
   if (a.n_walkers > 0)
-    prefetch(0);
+    prefetch(0, pre[0]);
+  if (DEPTH == 2 && a.n_walkers > 1)
+    prefetch(1, pre[DEPTH - 1]);
   for (int m = 0; m < a.n_walkers; ++m) {
-    __syncthreads();
+    auto stage = [&](C (&cur)[kBandPre]) {
+      __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kBandPre; ++j)
-      if (src[j] >= 0)
-        lds[dst[j]] = pre[j];
-    __syncthreads();
-    if (m + 1 < a.n_walkers)
-      prefetch(m + 1);
-    if (ok) {
-      const Real alpha = static_cast<Real>(0.5 * a.desc[2 * m].sign);
-      // Factors independent of b4.
-      C x31[2], y23[2], x13[2], y31[2];
-      C du{0, 0}, su{0, 0};
+      for (int j = 0; j < kBandPre; ++j)
+        if (src[j] >= 0)
+          lds[dst[j]] = cur[j];
+      __syncthreads();
+      if (m + DEPTH < a.n_walkers)
+        prefetch(m + DEPTH, cur);
+    };
+    if (DEPTH == 2 && (m & 1))
+      stage(pre[DEPTH - 1]);
+    else
+      stage(pre[0]);
+    if (!ok)
+      continue;
+    const Real alpha = static_cast<Real>(0.5 * a.desc[2 * m].sign);
+    // The LDS values are read in batches (the b4-independent factors, then per b4 its eight
+    // values) ahead of the products that use them: interleaving reads and products made each
+    // read wait for the one before.
+    C x31[2], y23[2], x13[2], y31[2];
+    C ga0{0, 0}, ga1{0, 0};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const C* X = Xl + s * bsz;
+      const C* Y = Yl + s * bsz;
+      if (kMag)
+        x31[s] = X[xi(b3, b1)];
+      if (kChg)
+        y23[s] = Y[yi(b2, b3)];
+      if (kPP)
+        x13[s] = X[xi(b1, b3)];
+      if (kTr)
+        y31[s] = Yl[(1 - s) * bsz + yi(b3, b1)];
+    }
+    if (kMag || kChg) {
+      ga0 = GAl[b2 + NB * (b1 + NB * k1)];
+      ga1 = GAl[nbk + b2 + NB * (b1 + NB * k1)];
+    }
+    const C du{alpha * (ga0.x - ga1.x), alpha * (ga0.y - ga1.y)};
+    const C su{alpha * (ga0.x + ga1.x), alpha * (ga0.y + ga1.y)};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (kMag)
+        x31[s] = C{alpha * x31[s].x, alpha * x31[s].y};
+      if (kChg)
+        y23[s] = C{alpha * y23[s].x, alpha * y23[s].y};
+      if (kPP)
+        x13[s] = C{alpha * x13[s].x, alpha * x13[s].y};
+      if (kTr)
+        y31[s] = C{alpha * y31[s].x, alpha * y31[s].y};
+    }
+#pragma unroll
+    for (int b4 = 0; b4 < NB; ++b4) {
+      C yb[2], xb[2], zb[2], xt[2], gb0{0, 0}, gb1{0, 0};
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
-        const C* X = Xl + s * bsz;
-        const C* Y = Yl + s * bsz;
         if (kMag)
-          x31[s] = X[xi(b3, b1)];
+          yb[s] = Yl[s * bsz + yi(b2, b4)];
         if (kChg)
-          y23[s] = Y[yi(b2, b3)];
+          xb[s] = Xl[s * bsz + xi(b1, b4)];
         if (kPP)
-          x13[s] = X[xi(b1, b3)];
+          zb[s] = Zl[(1 - s) * bsz + zi(b2, b4)];
         if (kTr)
-          y31[s] = Yl[(1 - s) * bsz + yi(b3, b1)];
+          xt[s] = Xl[s * bsz + xi(b2, b4)];
       }
       if (kMag || kChg) {
-        const C ga0 = GAl[b2 + NB * (b1 + NB * k1)], ga1 = GAl[nbk + b2 + NB * (b1 + NB * k1)];
-        du = C{alpha * (ga0.x - ga1.x), alpha * (ga0.y - ga1.y)};
-        su = C{alpha * (ga0.x + ga1.x), alpha * (ga0.y + ga1.y)};
+        gb0 = GBl[b3 + NB * (b4 + NB * k2)];
+        gb1 = GBl[nbk + b3 + NB * (b4 + NB * k2)];
       }
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        if (kMag) {
-          x31[s].x *= alpha;
-          x31[s].y *= alpha;
-        }
-        if (kChg) {
-          y23[s].x *= alpha;
-          y23[s].y *= alpha;
-        }
-        if (kPP) {
-          x13[s].x *= alpha;
-          x13[s].y *= alpha;
-        }
-        if (kTr) {
-          y31[s].x *= alpha;
-          y31[s].y *= alpha;
-        }
+      if (kMag) {
+        C t = cmul(du, C{gb0.x - gb1.x, gb0.y - gb1.y});
+        cfms(t, x31[0], yb[0]);
+        cfms(t, x31[1], yb[1]);
+        add(tot[b4][0], t.x, t.y);
       }
-      // Adds the walker's terms of entry b4 to the given partial sums.
-      auto terms = [&](int b4, C& am, C& ac, C& ap, C& at, C& adm, C& adc) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          const C* X = Xl + s * bsz;
-          if (kMag)
-            cfma(am, x31[s], Yl[s * bsz + yi(b2, b4)]);
-          if (kChg)
-            cfma(ac, X[xi(b1, b4)], y23[s]);
-          if (kPP)
-            cfma(ap, x13[s], Zl[(1 - s) * bsz + zi(b2, b4)]);
-          if (kTr)
-            cfma(at, X[xi(b2, b4)], y31[s]);
-        }
-        if (kMag || kChg) {
-          const C gb0 = GBl[b3 + NB * (b4 + NB * k2)], gb1 = GBl[nbk + b3 + NB * (b4 + NB * k2)];
-          if (kMag)
-            cfma(adm, du, C{gb0.x - gb1.x, gb0.y - gb1.y});
-          if (kChg)
-            cfma(adc, su, C{gb0.x + gb1.x, gb0.y + gb1.y});
-        }
-      };
-#pragma unroll
-      for (int b4 = 0; b4 < NB; ++b4) {
-        if constexpr (kChunk) {
-          terms(b4, xm[b4], xc[b4], pz[b4], tr[b4], dm[b4], dc[b4]);
-        }
-        else {
-          C t_m{0, 0}, t_c{0, 0}, t_p{0, 0}, t_t{0, 0}, t_dm{0, 0}, t_dc{0, 0};
-          terms(b4, t_m, t_c, t_p, t_t, t_dm, t_dc);
-          add_terms(b4, t_dm, t_m, t_dc, t_c, t_p, t_t);
-        }
+      if (kChg) {
+        C t = cmul(su, C{gb0.x + gb1.x, gb0.y + gb1.y});
+        cfms(t, xb[0], y23[0]);
+        cfms(t, xb[1], y23[1]);
+        add(tot[b4][1], t.x, t.y);
       }
-      if constexpr (kChunk) {
-        if ((m + 1) % FLUSH == 0 || m + 1 == a.n_walkers)
-          flush();
+      if (kPP) {
+        C t = cmul(x13[0], zb[0]);
+        cfma(t, x13[1], zb[1]);
+        add(tot[b4][2], t.x, t.y);
+      }
+      if (kTr) {
+        C t = cmul(xt[0], y31[0]);
+        cfma(t, xt[1], y31[1]);
+        add(tot[b4][3], -t.x, -t.y);
       }
     }
   }

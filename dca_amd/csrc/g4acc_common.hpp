@@ -47,6 +47,14 @@ __device__ __forceinline__ void cfma(C& acc, C a, C b) {
   acc.y = fma(a.x, b.y, acc.y);
   acc.y = fma(a.y, b.x, acc.y);
 }
+// acc -= a * b
+template <typename C>
+__device__ __forceinline__ void cfms(C& acc, C a, C b) {
+  acc.x = fma(-a.x, b.x, acc.x);
+  acc.x = fma(a.y, b.y, acc.x);
+  acc.y = fma(-a.x, b.y, acc.y);
+  acc.y = fma(-a.y, b.x, acc.y);
+}
 // acc += a * conj(b)
 template <typename C>
 __device__ __forceinline__ void cfmac(C& acc, C a, C b) {
