@@ -22,6 +22,7 @@ Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import platform
@@ -58,6 +59,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-calibration", type=int, default=0,
                    help="bytes of a plain device-to-device copy run first (counter calibration)")
+    p.add_argument("--fixed-n", type=int, default=0,
+                   help="every measurement of the headline gets this vertex count per spin instead "
+                        "of the workload's law (stage times against N; at most the workload's N_max)")
     return p.parse_args()
 
 
@@ -133,13 +137,18 @@ def mtog_flops(g, n):
             + 12 * D * D)              # G = -G0 M~ G0 + beta G0 delta
 
 
-def time_config(cfg, steps, warmup, walkers, pool_n, plain_steps, torch, keep_sample=0):
+def time_config(cfg, steps, warmup, walkers, pool_n, plain_steps, torch, keep_sample=0, fixed_n=0):
     """Times the mixed (batched) and plain forms of one workload; returns a dict of results and,
     if keep_sample > 0, the first keep_sample measurements with both forms' G4 over them."""
     from dca_amd import g4acc, synth
     from dca_amd.devpool import DevicePool
     from oracle import bruteforce as bf
     wl = synth.workload(cfg)
+    if fixed_n:
+        if not 0 < fixed_n <= wl.n_max:
+            raise SystemExit(f"--fixed-n must be in [1, {wl.n_max}] for workload {cfg}")
+        wl = dataclasses.replace(wl, draw_n=lambda rs: fixed_n,
+                                 description=f"{wl.description}; N fixed at {fixed_n}")
     g = wl.geometry
     W = walkers or wl.walkers
     log(f"workload {cfg}: {wl.description}; walkers/batch={W}")
@@ -345,7 +354,7 @@ def main():
     keep = 0 if args.no_cpu else threads
 
     head, ctx, sample = time_config(args.config, args.steps, args.warmup, args.walkers, args.pool,
-                                    args.plain_steps, torch, keep_sample=keep)
+                                    args.plain_steps, torch, keep_sample=keep, fixed_n=args.fixed_n)
     roof = roofline(args.config, ctx)
 
     extras = {}
