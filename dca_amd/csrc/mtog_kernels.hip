@@ -205,8 +205,19 @@ __global__ void __launch_bounds__(256) dft_right_G_kernel(Geo g, int slot0,
   const int slot = slot0 + blockIdx.z;
   const int s = slot & 1;
   const int KW = g.nc * g.nwE;
-  const int kw1 = blockIdx.x * blockDim.x + threadIdx.x;
-  const int kw2 = blockIdx.y;
+  // Workgroups are dealt to the 8 XCDs round-robin by id; a slot's grid is remapped so that each
+  // XCD takes a contiguous range of (kw1 block, kw2), and the Nc workgroups of one frequency w2,
+  // which read the same columns of E, share its L2.
+  unsigned bx = blockIdx.x, by = blockIdx.y;
+  const unsigned per_slot = gridDim.x * gridDim.y;
+  if (per_slot % 8 == 0) {
+    const unsigned lin = blockIdx.x + gridDim.x * blockIdx.y;
+    const unsigned lid = (lin % 8) * (per_slot / 8) + lin / 8;
+    bx = lid % gridDim.x;
+    by = lid / gridDim.x;
+  }
+  const int kw1 = static_cast<int>(bx * blockDim.x + threadIdx.x);
+  const int kw2 = static_cast<int>(by);
   if (kw1 >= KW)
     return;
   const int k1 = kw1 % g.nc, w1 = kw1 / g.nc;
