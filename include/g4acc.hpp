@@ -197,9 +197,21 @@ public:
   double accumulate(const MatrixPair& M_pair, const std::array<Configuration, 2>& configs,
                     const Sign sign, int memory = G4ACC_HOST) {
     using Scalar = std::remove_cv_t<std::remove_pointer_t<decltype(M_pair[0].ptr())>>;
-    if (model_.non_density_density || model_.complex_m || !model_.spin_symmetric ||
-        is_complex<Scalar>::value || is_complex<Sign>::value)
+    if constexpr (is_complex<Scalar>::value || is_complex<Sign>::value) {
       return accumulate_general(M_pair, configs, sign, memory);
+    }
+    else {
+      if (model_.non_density_density || model_.complex_m || !model_.spin_symmetric)
+        return accumulate_general(M_pair, configs, sign, memory);
+      return accumulate_real(M_pair, configs, sign, memory);
+    }
+  }
+
+ private:
+  // The density-density, spin-symmetric, real model (g4acc_accumulate / g4acc_accumulate_queued).
+  template <class MatrixPair, class Configuration, class Sign>
+  double accumulate_real(const MatrixPair& M_pair, const std::array<Configuration, 2>& configs,
+                         const Sign sign, int memory) {
     g4acc_walker w{};
     for (int s = 0; s < 2; ++s) {
       const auto& conf = configs[s];
@@ -218,6 +230,7 @@ public:
     return flops(w.n[0], w.n[1]);
   }
 
+ public:
   // accumulate for the general model (g4acc_accumulate_general): complex M and sign, and the
   // vertices' left labels when the model is non-density-density.
   template <class MatrixPair, class Configuration, class Sign>
@@ -236,19 +249,25 @@ public:
         extract_configuration(conf, tau_[s], site_[s], band_[s]);
       w.n[s] = n;
       w.ld[s] = n ? M_pair[s].leadingDimension() : 1;
-      if (model_.complex_m && !is_complex<Scalar>::value && n) {
-        // Real M on a complex handle: widened into a complex copy (column-major, leading dimension n).
-        mc_[s].assign(2 * static_cast<std::size_t>(n) * n, 0.);
-        const Scalar* m = M_pair[s].ptr();
-        for (int j = 0; j < n; ++j)
-          for (int i = 0; i < n; ++i)
-            mc_[s][2 * (i + static_cast<std::size_t>(n) * j)] = static_cast<double>(m[i + static_cast<std::size_t>(w.ld[s]) * j]);
-        if (memory != G4ACC_HOST)
-          throw std::logic_error("g4acc: a complex handle takes real device M only as complex entries");
-        w.m[s] = mc_[s].data();
-        w.ld[s] = n;
+      bool widened = false;
+      if constexpr (!is_complex<Scalar>::value) {
+        if (model_.complex_m && n) {
+          // Real M on a complex handle: widened into a complex copy (column-major, leading
+          // dimension n), on the host only.
+          if (memory != G4ACC_HOST)
+            throw std::logic_error("g4acc: a complex handle takes real device M only as complex entries");
+          mc_[s].assign(2 * static_cast<std::size_t>(n) * n, 0.);
+          const Scalar* m = M_pair[s].ptr();
+          for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+              mc_[s][2 * (i + static_cast<std::size_t>(n) * j)] =
+                  static_cast<double>(m[i + static_cast<std::size_t>(w.ld[s]) * j]);
+          w.m[s] = mc_[s].data();
+          w.ld[s] = n;
+          widened = true;
+        }
       }
-      else
+      if (!widened)
         w.m[s] = n ? reinterpret_cast<const double*>(M_pair[s].ptr()) : nullptr;
       w.tau[s] = tau_[s].data();
       w.site[s] = site_[s].data();

@@ -38,6 +38,38 @@ def write_case(path, wl, measurements, expected):
             np.asarray(e, np.complex128).tofile(f)
 
 
+def write_case_general(path, g, model, measurements, expected):
+    """The general-model case file of tests/cpp/tp_accumulator_general_test.cpp: the geometry,
+    the model flags (spin_symmetric, complex_m, non_density_density), then per measurement n[2],
+    the complex sign, and per spin M (complex128 or float64, column-major), tau, site, band and,
+    for non-density-density models, the left site and band; then the expected G4 per channel."""
+    ss, cm, nd = model
+    with open(path, "wb") as f:
+        np.array([g.nb, g.nc, g.n_freq, len(g.w_ex), len(g.k_ex), len(g.channels),
+                  len(measurements), g.g4_size(), int(ss), int(cm), int(nd)], np.int32).tofile(f)
+        np.array([g.beta], np.float64).tofile(f)
+        for a in (g.w_ex, g.k_ex, g.channels):
+            np.asarray(a, np.int32).tofile(f)
+        np.asarray(g.cluster.add.ravel(order="F"), np.int32).tofile(f)
+        np.asarray(g.cluster.sub.ravel(order="F"), np.int32).tofile(f)
+        np.asarray(g.cluster.dft.ravel(order="F"), np.complex128).tofile(f)
+        geo.g0_flat(g.g0).tofile(f)
+        for m in measurements:
+            np.array(m.n, np.int32).tofile(f)
+            z = complex(m.sign)
+            np.array([z.real, z.imag], np.float64).tofile(f)
+            for s in range(2):
+                np.asarray(m.M[s], np.complex128 if cm else np.float64).ravel(order="F").tofile(f)
+                np.asarray(m.tau[s], np.float64).tofile(f)
+                np.asarray(m.site[s], np.int32).tofile(f)
+                np.asarray(m.band[s], np.int32).tofile(f)
+                if nd:
+                    np.asarray(m.site_l[s], np.int32).tofile(f)
+                    np.asarray(m.band_l[s], np.int32).tofile(f)
+        for e in expected:
+            np.asarray(e, np.complex128).tofile(f)
+
+
 def test_case_file_layout(tmp_path):
     """The case writer produces exactly what the C++ reader consumes (CPU check of the format)."""
     wl = synth.tiny_workload(nb=1, L=2, n_freq=4, w_ex=(0,), k_ex=((0.0, 0.0),), n_range=(1, 3))
@@ -93,6 +125,39 @@ def test_cpp_mirror_matches_oracle(oracle_lib, tmp_path, nb, L, precision, tol):
     write_case(p, wl, ms, ref.get_G4())
     print(f"C++ mirror nb={nb} L={L} precision={precision}", flush=True)
     r = subprocess.run([BIN, str(p), str(precision), str(tol)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0 and "OK" in r.stdout
+
+
+GENERAL_BIN = os.path.join(HERE, "cpp", "_build", "tp_accumulator_general_test")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nb,L,model", [(2, 2, (1, 1, 1)), (1, 3, (1, 1, 0)), (2, 2, (0, 1, 1)),
+                                        (3, 2, (0, 0, 0))])
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-6)])
+def test_cpp_mirror_general_model(oracle_lib, tmp_path, nb, L, model, precision, tol):
+    """The C++ mirror on the general model (complex M and signs, left labels apart from the right
+    ones, not spin symmetric) through g4acc_accumulate_general, against the FP64 oracle."""
+    assert os.path.exists(GENERAL_BIN), "build the driver with tests/cpp/build.sh (__graft_entry__.build())"
+    ss, cm, nd = model
+    channels = ([geo.PARTICLE_HOLE_TRANSVERSE, geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_HOLE_CHARGE,
+                 geo.PARTICLE_PARTICLE_UP_DOWN] if ss else
+                [geo.PARTICLE_HOLE_MAGNETIC, geo.PARTICLE_PARTICLE_UP_DOWN])
+    g = synth.tiny_workload(nb=nb, L=L, n_freq=4, w_ex=(0, 1), k_ex=((0.0, 0.0), (2 * math.pi / L, 0.0)),
+                            channels=channels, zeeman=0.23).geometry
+    rs = synth.Stream(700 + 10 * nb + L + 3 * ss + 5 * cm + 7 * nd)
+    ms = [synth.general_measurement(rs, g.nb, g.nc, g.beta, (rs.randint(3, 14), rs.randint(3, 14)),
+                                    complex_m=bool(cm), non_density=bool(nd), phase=0.4 + 1.3 * i)
+          for i in range(4)]
+    ref = oracle_lib.OracleAccumulator(g, spin_symmetric=bool(ss))
+    for m in ms:
+        ref.accumulate(m)
+    p = tmp_path / "case.bin"
+    write_case_general(p, g, model, ms, ref.get_G4())
+    print(f"C++ mirror general nb={nb} L={L} model={model} precision={precision}", flush=True)
+    r = subprocess.run([GENERAL_BIN, str(p), str(precision), str(tol)], capture_output=True, text=True,
                        timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0 and "OK" in r.stdout
