@@ -146,11 +146,15 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
   //   MAGNETIC = dm - xs, CHARGE = dc - xs, PP = pz, TRANSVERSE = -tr,
   // and the FP64 additions happen once per FLUSH walkers. With complex128 G every walker's
   // channel term goes straight into the totals.
-  constexpr bool kChunk = sizeof(Real) == 4;
+  // 1024-thread tiles of two pairs per thread with precomputed rank-one terms (6x6 batches): each
+  // walker's terms go straight into the FP64 totals, and the registers of the FP32 partials hold
+  // the next walker's X instead, so the walker loop waits on no global load.
+  constexpr bool kDirect = RX && TT == 1024 && PPT == 2;
+  constexpr bool kChunk = sizeof(Real) == 4 && !kDirect;
   constexpr int FLUSH = 8;
   constexpr int PC = kChunk ? PPT : 1, QC = kChunk ? QG : 1;
   // Registers: with many pairs per thread (or 1024-thread workgroups, 128 VGPRs) X is read in place.
-  constexpr bool kPrefetchX = PPT * (TT / 256) <= 4 && !(TT > 256 && PPT > 1);
+  constexpr bool kPrefetchX = kDirect || (PPT * (TT / 256) <= 4 && !(TT > 256 && PPT > 1));
   bool has_trans = a.has_trans, has_mag = false, has_chg = false;
 #pragma unroll
   for (int c = 0; c < NCH; ++c) {
@@ -307,11 +311,13 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
             C t;
             if (ch == PH_MAGNETIC) {
               t = csub(C{0, 0}, cross);
-              cfma(t, uq[k1[j]].a, uq[QG * nc + k2[j]].a);
+              if (!RX)
+                cfma(t, uq[k1[j]].a, uq[QG * nc + k2[j]].a);
             }
             else if (ch == PH_CHARGE) {
               t = csub(C{0, 0}, cross);
-              cfma(t, uq[k1[j]].b, uq[QG * nc + k2[j]].b);
+              if (!RX)
+                cfma(t, uq[k1[j]].b, uq[QG * nc + k2[j]].b);
             }
             else if (ch == PP_UP_DOWN)
               t = ppz;
