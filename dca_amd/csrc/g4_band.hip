@@ -121,38 +121,36 @@ __global__ void __launch_bounds__(kBandThreads) g4_band_kernel(TiledArgs<Real> a
     t.y += static_cast<double>(y);
   };
 
-  // Staged values of walker m + DEPTH are loaded while walker m is consumed (DEPTH register sets:
-  // two for complex64 G, one for complex128, whose second set would not fit in registers).
+  // Staged values of walkers m + 1 .. m + DEPTH are loaded while walker m is consumed (DEPTH
+  // register sets: two for complex64 G, one for complex128, whose second set would not fit in
+  // registers). Every load is issued unconditionally (idle staging slots re-read element 0, walkers
+  // past the batch re-read the last one) and the loop is unrolled by DEPTH, so the loads in flight
+  // are counted exactly and staging waits only for the walker it writes.
   constexpr int DEPTH = sizeof(Real) == 4 ? 2 : 1;
+  const int n_walk = a.n_walkers;
+  bool use[kBandPre];
+#pragma unroll
+  for (int j = 0; j < kBandPre; ++j) {
+    use[j] = src[j] >= 0;
+    if (!use[j])
+      src[j] = 0;
+  }
   C pre[DEPTH][kBandPre];
   auto prefetch = [&](int m, C (&pr)[kBandPre]) {
-    const C* H = a.G + (2 * static_cast<size_t>(m)) * DD;
+    const C* H = a.G + (2 * static_cast<size_t>(m < n_walk ? m : n_walk - 1)) * DD;
 #pragma unroll
     for (int j = 0; j < kBandPre; ++j)
-      if (src[j] >= 0)
-        pr[j] = H[src[j]];
+      pr[j] = H[src[j]];
   };
-  if (a.n_walkers > 0)
-    prefetch(0, pre[0]);
-  if (DEPTH == 2 && a.n_walkers > 1)
-    prefetch(1, pre[DEPTH - 1]);
-  for (int m = 0; m < a.n_walkers; ++m) {
-    auto stage = [&](C (&cur)[kBandPre]) {
-      __syncthreads();
+  auto stage = [&](const C (&cur)[kBandPre]) {
+    __syncthreads();
 #pragma unroll
-      for (int j = 0; j < kBandPre; ++j)
-        if (src[j] >= 0)
-          lds[dst[j]] = cur[j];
-      __syncthreads();
-      if (m + DEPTH < a.n_walkers)
-        prefetch(m + DEPTH, cur);
-    };
-    if (DEPTH == 2 && (m & 1))
-      stage(pre[DEPTH - 1]);
-    else
-      stage(pre[0]);
-    if (!ok)
-      continue;
+    for (int j = 0; j < kBandPre; ++j)
+      if (use[j])
+        lds[dst[j]] = cur[j];
+    __syncthreads();
+  };
+  auto consume = [&](int m) {
     const Real alpha = static_cast<Real>(0.5 * a.desc[2 * m].sign);
     // The LDS values are read in batches (the b4-independent factors, then per b4 its eight
     // values) ahead of the products that use them: interleaving reads and products made each
@@ -228,6 +226,20 @@ __global__ void __launch_bounds__(kBandThreads) g4_band_kernel(TiledArgs<Real> a
         C t = cmul(xt[0], y31[0]);
         cfma(t, xt[1], y31[1]);
         add(tot[b4][3], -t.x, -t.y);
+      }
+    }
+  };
+  if (n_walk > 0) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; ++d)
+      prefetch(d, pre[d]);
+    for (int m = 0; m < n_walk; m += DEPTH) {
+#pragma unroll
+      for (int d = 0; d < DEPTH; ++d) {
+        stage(pre[d]);
+        prefetch(m + d + DEPTH, pre[d]);
+        if (ok && m + d < n_walk)
+          consume(m + d);
       }
     }
   }
