@@ -146,15 +146,15 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
   //   MAGNETIC = dm - xs, CHARGE = dc - xs, PP = pz, TRANSVERSE = -tr,
   // and the FP64 additions happen once per FLUSH walkers. With complex128 G every walker's
   // channel term goes straight into the totals.
-  // 1024-thread tiles of two pairs per thread with precomputed rank-one terms (6x6 batches): each
-  // walker's terms go straight into the FP64 totals, and the registers of the FP32 partials hold
-  // the next walker's X instead, so the walker loop waits on no global load.
-  constexpr bool kDirect = RX && TT == 1024 && PPT == 2;
-  constexpr bool kChunk = sizeof(Real) == 4 && !kDirect;
+  // 1024-thread tiles of two pairs per thread with precomputed rank-one terms (6x6 batches) run a
+  // two-walker pipeline (below): each walker's terms go straight into the FP64 totals, whose
+  // registers would otherwise hold FP32 partials.
+  constexpr bool kPipe = RX && TT == 1024 && PPT == 2;
+  constexpr bool kChunk = sizeof(Real) == 4 && !kPipe;
   constexpr int FLUSH = 8;
   constexpr int PC = kChunk ? PPT : 1, QC = kChunk ? QG : 1;
   // Registers: with many pairs per thread (or 1024-thread workgroups, 128 VGPRs) X is read in place.
-  constexpr bool kPrefetchX = kDirect || (PPT * (TT / 256) <= 4 && !(TT > 256 && PPT > 1));
+  constexpr bool kPrefetchX = PPT * (TT / 256) <= 4 && !(TT > 256 && PPT > 1);
   bool has_trans = a.has_trans, has_mag = false, has_chg = false;
 #pragma unroll
   for (int c = 0; c < NCH; ++c) {
@@ -180,6 +180,95 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
 #pragma unroll
     for (int qi = 0; qi < QC; ++qi)
       xs[j][qi] = pz[j][qi] = tr[j][qi] = dm[j][qi] = dc[j][qi] = C{0, 0};
+  if constexpr (kPipe) {
+    // X, Y and Z (both spins) of walkers m + 1 and m + 2 are in flight while walker m is consumed.
+    // The elements a walker stages form one flat list, PS per thread, each stored with both spins
+    // in one 16-byte write; their source offsets (within the walker's G, below 2^31) and LDS
+    // places are fixed for the tile. Every load is issued (idle slots re-read element 0, walkers
+    // past the batch re-read the last one) and the loop is unrolled by two, so the loads in
+    // flight are counted exactly and staging waits only for the walker it writes.
+    constexpr int PS = 4;  // 3 nc^2 <= PS TT
+    constexpr int NBK = 1 + ((MASK & kY) != 0) + ((MASK & (1 << PP_UP_DOWN)) != 0);
+    P2* Xl = Yl + (NBK - 1) * bs;  // after Y and Z
+    const int ncc = nc * nc, n_items = NBK * ncc, n_walk = a.n_walkers;
+    int psrc[PS], pdst[PS];
+    bool puse[PS];
+#pragma unroll
+    for (int j = 0; j < PS; ++j) {
+      const int f = threadIdx.x + TT * j;
+      puse[j] = f < n_items;
+      const int blk = puse[j] ? f / ncc : 0, idx = f % ncc;
+      // Blocks in LDS order: Y (if staged), Z (if staged), X.
+      const bool isx = blk == NBK - 1, isy = !isx && needY && blk == 0;
+      const size_t boff = isx ? xoff : (isy ? yoff : zoff);
+      psrc[j] = puse[j] ? static_cast<int>(boff + (idx % nc) + D * (idx / nc)) : 0;
+      pdst[j] = blk * bs + (idx / nc) * ld + idx % nc;
+    }
+    C pre[2][PS][2];
+    auto fetch = [&](int m, C (&pr)[PS][2]) {
+      const C* H = a.G + (2 * static_cast<size_t>(m < n_walk ? m : n_walk - 1)) * DD;
+#pragma unroll
+      for (int j = 0; j < PS; ++j) {
+        pr[j][0] = H[psrc[j]];
+        pr[j][1] = H[DD + psrc[j]];
+      }
+    };
+    auto stage = [&](const C (&cur)[PS][2]) {
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < PS; ++j)
+        if (puse[j])
+          store_pair(Yl[pdst[j]], cur[j][0], cur[j][1]);
+      __syncthreads();
+    };
+    auto consume = [&](int m) {
+      const Real alpha = static_cast<Real>(0.5 * a.desc[2 * m].sign);
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        if (!ok[j])
+          continue;
+        const P2 xx = Xl[k2[j] * ld + k1[j]];
+        const C x0{alpha * xx.a.x, alpha * xx.a.y}, x1{alpha * xx.b.x, alpha * xx.b.y};
+#pragma unroll
+        for (int qi = 0; qi < QG; ++qi) {
+          if (!qok[qi])
+            continue;
+          C cross{0, 0}, ppz{0, 0};
+          if (needY) {
+            const P2 yy = Yl[yi[j][qi]];
+            cross = cmul(x0, yy.a);
+            cfma(cross, x1, yy.b);
+          }
+          if (needZ) {
+            const P2 zz = Zl[zi[j][qi]];
+            ppz = cmul(x0, zz.b);
+            cfma(ppz, x1, zz.a);
+          }
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) {
+            const int ch = a.ch[c];
+            const C t = ch == PP_UP_DOWN ? ppz : C{-cross.x, -cross.y};
+            tot[j][qi][c].x += static_cast<double>(t.x);
+            tot[j][qi][c].y += static_cast<double>(t.y);
+          }
+        }
+      }
+    };
+    if (n_walk > 0) {
+      fetch(0, pre[0]);
+      fetch(1, pre[1]);
+      for (int m = 0; m < n_walk; m += 2) {
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          stage(pre[d]);
+          fetch(m + d + 2, pre[d]);
+          if (m + d < n_walk)
+            consume(m + d);
+        }
+      }
+    }
+  }
+  else {
   // Everything walker m needs is loaded into registers while walker m - 1 is consumed: the
   // staged blocks (each thread stages at most PPT elements per block, nc^2 <= 256 PPT), the
   // rank-one factors and (for few pairs per thread) the thread's own X entries.
@@ -355,6 +444,7 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
       }
     }
   }
+  }
   auto store = [&](int c, size_t idx, double2 v) {
     if constexpr (RX) {
       if (a.r1_slot[c] >= 0) {
@@ -419,6 +509,12 @@ void launch_tiled_fixed(TiledArgs<Real> a, size_t lds, hipStream_t st) {
   if constexpr (kRX) {
     if (a.R) {
       launch_g4_rank1(a, st);
+      if constexpr (TT == 1024 && PPT == 2) {
+        // The pipelined tile stages X beside Y and Z (both spins each).
+        constexpr int kY = (1 << PH_MAGNETIC) | (1 << PH_CHARGE) | (1 << PH_TRANSVERSE);
+        constexpr int NBK = 1 + ((MASK & kY) != 0) + ((MASK & (1 << PP_UP_DOWN)) != 0);
+        lds = sizeof(Cplx<Real>) * 2 * NBK * static_cast<size_t>(a.g.nc) * (a.g.nc + 1);
+      }
       hipLaunchKernelGGL((g4_tiled_kernel<Real, TT, PPT, QG, NCH, MASK, true>), dim3(tiles), dim3(TT), lds, st, a);
       return;
     }
@@ -533,6 +629,9 @@ bool launch_g4_tiled(TiledArgs<Real> a, hipStream_t st) {
   for (int c = 0; c < a.n_ch; ++c)
     mask |= 1 << a.ch[c];
   constexpr int kM = 1 << PH_MAGNETIC, kQ = 1 << PH_CHARGE, kP = 1 << PP_UP_DOWN;
+  // The pipelined 1024 x 2 tile addresses a walker's G with 32-bit offsets.
+  if (ts.threads == 1024 && ppt == 2 && 2 * static_cast<size_t>(g.D) * g.D >= (size_t(1) << 31))
+    a.R = nullptr;
   const bool rx = std::is_same<Real, float>::value && a.R != nullptr &&
                   (mask == (kM | kQ | kP) || mask == (kM | kP) || mask == kQ || mask == kM);
   const int acc_budget = ts.threads == 256 ? 16 : (rx ? 8 : 4);
