@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
     // places are fixed for the tile. Every load is issued (idle slots re-read element 0, walkers
     // past the batch re-read the last one) and the loop is unrolled by two, so the loads in
     // flight are counted exactly and staging waits only for the walker it writes.
-    constexpr int PS = 4;  // 3 nc^2 <= PS TT
+    constexpr int PS = 4;  // 3 nc^2 <= PS TT (the launcher takes this path for Nc <= 36 only)
     constexpr int NBK = 1 + ((MASK & kY) != 0) + ((MASK & (1 << PP_UP_DOWN)) != 0);
     P2* Xl = Yl + (NBK - 1) * bs;  // after Y and Z
     const int ncc = nc * nc, n_items = NBK * ncc, n_walk = a.n_walkers;
@@ -629,8 +629,10 @@ bool launch_g4_tiled(TiledArgs<Real> a, hipStream_t st) {
   for (int c = 0; c < a.n_ch; ++c)
     mask |= 1 << a.ch[c];
   constexpr int kM = 1 << PH_MAGNETIC, kQ = 1 << PH_CHARGE, kP = 1 << PP_UP_DOWN;
-  // The pipelined 1024 x 2 tile addresses a walker's G with 32-bit offsets.
-  if (ts.threads == 1024 && ppt == 2 && 2 * static_cast<size_t>(g.D) * g.D >= (size_t(1) << 31))
+  // The pipelined 1024 x 2 tile (the rank-one path of that shape) addresses a walker's G with
+  // 32-bit offsets and stages at most 4 x 1024 elements per walker (3 nc^2: Nc <= 36).
+  if (ts.threads == 1024 && ppt == 2 &&
+      (2 * static_cast<size_t>(g.D) * g.D >= (size_t(1) << 31) || 3 * g.nc * g.nc > 4 * 1024))
     a.R = nullptr;
   const bool rx = std::is_same<Real, float>::value && a.R != nullptr &&
                   (mask == (kM | kQ | kP) || mask == (kM | kP) || mask == kQ || mask == kM);
