@@ -59,6 +59,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-calibration", type=int, default=0,
                    help="bytes of a plain device-to-device copy run first (counter calibration)")
+    p.add_argument("--sum-mode", default="fp64", choices=("fp64", "fp32_kahan"),
+                   help="G4 running sum of the mixed form (the plain form always sums in FP64)")
     p.add_argument("--fixed-n", type=int, default=0,
                    help="every measurement of the headline gets this vertex count per spin instead "
                         "of the workload's law (stage times against N; at most the workload's N_max)")
@@ -137,7 +139,8 @@ def mtog_flops(g, n):
             + 12 * D * D)              # G = -G0 M~ G0 + beta G0 delta
 
 
-def time_config(cfg, steps, warmup, walkers, pool_n, plain_steps, torch, keep_sample=0, fixed_n=0):
+def time_config(cfg, steps, warmup, walkers, pool_n, plain_steps, torch, keep_sample=0, fixed_n=0,
+                sum_mode="fp64"):
     """Times the mixed (batched) and plain forms of one workload; returns a dict of results and,
     if keep_sample > 0, the first keep_sample measurements with both forms' G4 over them."""
     from dca_amd import g4acc, synth
@@ -158,7 +161,7 @@ def time_config(cfg, steps, warmup, walkers, pool_n, plain_steps, torch, keep_sa
     dp = DevicePool(torch, pool)
     batches = dp.batches(W)
 
-    mixed = g4acc.TpAccumulator(g, precision="fp32", sum_mode="fp64", max_walkers=W,
+    mixed = g4acc.TpAccumulator(g, precision="fp32", sum_mode=sum_mode, max_walkers=W,
                                 max_vertices=wl.n_max)
     mixed.resetAccumulation()
     run_batches(mixed, batches, warmup)
@@ -199,7 +202,7 @@ def time_config(cfg, steps, warmup, walkers, pool_n, plain_steps, torch, keep_sa
 
     # Mixed vs plain over the first batch of measurements (the plain form is pinned to the oracle
     # at ~1e-15 by the parity tests).
-    acc_m = g4acc.TpAccumulator(g, precision="fp32", sum_mode="fp64", max_walkers=W,
+    acc_m = g4acc.TpAccumulator(g, precision="fp32", sum_mode=sum_mode, max_walkers=W,
                                 max_vertices=wl.n_max)
     acc_p = g4acc.TpAccumulator(g, precision="fp64", sum_mode="fp64", max_walkers=1,
                                 max_vertices=wl.n_max)
@@ -354,14 +357,15 @@ def main():
     keep = 0 if args.no_cpu else threads
 
     head, ctx, sample = time_config(args.config, args.steps, args.warmup, args.walkers, args.pool,
-                                    args.plain_steps, torch, keep_sample=keep, fixed_n=args.fixed_n)
+                                    args.plain_steps, torch, keep_sample=keep, fixed_n=args.fixed_n,
+                                    sum_mode=args.sum_mode)
     roof = roofline(args.config, ctx)
 
     extras = {}
     for c in [int(x) for x in args.extra.split(",") if x.strip()]:
         if c == args.config:
             continue
-        res, cctx, _ = time_config(c, args.extra_steps, 2, 0, 2, 0, torch)
+        res, cctx, _ = time_config(c, args.extra_steps, 2, 0, 2, 0, torch, sum_mode=args.sum_mode)
         res["roofline"] = roofline(c, cctx)
         extras[f"cfg{c}"] = res
 
@@ -400,7 +404,7 @@ def main():
             "workload": head["workload"],
             "walkers_per_step": W,
             "g_precision": "complex64 (FP32 products, FP32 accumulation)",
-            "g4_sum": "fp64",
+            "g4_sum": args.sum_mode,
             "plain_form": "complex128 G, FP64 G4, 1 measurement per launch sequence",
         },
         "plain_fp64_value": head["plain_fp64_value"],
