@@ -124,6 +124,7 @@ __global__ void __launch_bounds__(256) ndft_left_kernel(const SlotDesc* __restri
   C acc{0, 0};
   if (d.m_cplx) {
     const double2* mcol = reinterpret_cast<const double2*>(d.m) + static_cast<size_t>(d.ld) * j;
+#pragma unroll 4
     for (int ii = O[o]; ii < O[o + 1]; ++ii) {
       const double2 m = mcol[P[ii]];
       cfma(acc, Ts[w + static_cast<size_t>(g.nwE) * ii], C{static_cast<Real>(m.x), static_cast<Real>(m.y)});
@@ -132,6 +133,7 @@ __global__ void __launch_bounds__(256) ndft_left_kernel(const SlotDesc* __restri
   else {
     const double* mcol = d.m + static_cast<size_t>(d.ld) * j;
     const double* xv = d.expv;  // N and exp(V): M_ij = (exp(V_i) - 1) N_ij, formed in FP64
+#pragma unroll 4
     for (int ii = O[o]; ii < O[o + 1]; ++ii) {
       const int i = P[ii];
       const Real m = static_cast<Real>(xv ? (xv[i] - 1.0) * mcol[i] : mcol[i]);
@@ -160,6 +162,7 @@ __global__ void __launch_bounds__(256) dft_left_kernel(const SlotDesc* __restric
   const size_t base = static_cast<size_t>(blockIdx.z) * g.D * n_max + static_cast<size_t>(g.D) * jj;
   const C* Bc = B + base + b + static_cast<size_t>(g.n_orb) * w;
   C acc{0, 0};
+#pragma unroll 8
   for (int r = 0; r < g.nc; ++r)
     cfma(acc, dft[k + g.nc * r], Bc[g.nb * r]);
   A[base + p] = acc;
@@ -185,6 +188,7 @@ __global__ void __launch_bounds__(256) ndft_right_kernel(const SlotDesc* __restr
   const C* Ts = T + static_cast<size_t>(slot) * g.nwE * n_max;
   const C* As = A + static_cast<size_t>(blockIdx.z) * g.D * n_max;
   C acc{0, 0};
+#pragma unroll 4
   for (int jj = O[o2]; jj < O[o2 + 1]; ++jj)
     cfmac(acc, As[p1 + static_cast<size_t>(g.D) * jj], Ts[w2 + static_cast<size_t>(g.nwE) * jj]);
   E[static_cast<size_t>(blockIdx.z) * g.D * g.D + p1 + static_cast<size_t>(g.D) * q2] = acc;
@@ -231,6 +235,7 @@ __global__ void __launch_bounds__(256) dft_right_G_kernel(Geo g, int slot0,
 #pragma unroll
     for (int f = 0; f < NB; ++f) {
       C acc{0, 0};
+#pragma unroll 8
       for (int r2 = 0; r2 < g.nc; ++r2)
         cfmac(acc, Es[row + static_cast<size_t>(g.D) * (f + NB * r2 + static_cast<size_t>(g.n_orb) * w2)],
               dft[k2 + g.nc * r2]);
