@@ -68,6 +68,7 @@ struct g4acc_handle {
   int precision = G4ACC_FP64;
   int sum_mode = G4ACC_SUM_FP64;
   int max_walkers = 1, max_vertices = 1, chunk = 1;
+  int mtog_chunk = 0;  // slots per M -> G pass when below chunk (tuning key; 0: chunk)
   std::vector<int> channels, w_ex, k_ex;
   size_t g4_entries = 0;   // per channel
   bool initialized = false;
@@ -472,7 +473,7 @@ MtoGArgs<Real> mtog_args(g4acc_handle* h, int n_slots) {
   a.desc = h->d_desc;
   a.n_slots = n_slots;
   a.n_max = h->max_vertices;
-  a.chunk = h->chunk;
+  a.chunk = h->mtog_chunk > 0 && h->mtog_chunk < h->chunk ? h->mtog_chunk : h->chunk;
   a.fused = h->fused_mtog;
   a.tw = h->mtog_tw;
   a.mfma = h->mtog_mfma;
@@ -843,6 +844,10 @@ int g4acc_create_model(const g4acc_geometry* gm, const g4acc_options* opt, const
                           : h->large_ok  ? cb * (2 * D * NM + (h->large_e ? D * D : 0)) + cb / 2 * NM * NM
                                          : cb * D * NM;
   h->chunk = static_cast<int>(std::max<size_t>(1, std::min<size_t>(W2, workspace_budget_bytes() / per_slot)));
+  // A whole batch that fits 1.5x the budget runs as one chunk: a tail chunk of a few slots leaves
+  // most of the GPU idle in the left-half kernels.
+  if (static_cast<size_t>(h->chunk) < W2 && W2 * per_slot <= workspace_budget_bytes() / 2 * 3)
+    h->chunk = static_cast<int>(W2);
   const size_t CH = h->chunk;
 
   if ((rc = h->alloc(&h->d_kadd, nc2, "k_add")) || (rc = h->alloc(&h->d_ksub, nc2, "k_sub")) ||
@@ -1540,6 +1545,8 @@ int g4acc_set_tuning(g4acc_handle* h, const char* key, int value) {
   else if (k == "mtog_large" &&
            (value == 0 || (value == 1 && h->large_e) || (value == 2 && h->right_fused_ok)))
     h->mtog_large = value;
+  else if (k == "mtog_chunk" && value >= 0)
+    h->mtog_chunk = value;
   else if (k == "g4_sym" && (value == 0 || (value == 1 && h->sym_ok)))
     h->g4_sym = value;
   else if (k == "g4_qg" && (value == 0 || value == 1 || value == 2 || value == 4))

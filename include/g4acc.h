@@ -281,8 +281,10 @@ int g4acc_set_timing(g4acc_handle* h, int enable);
  * 0 register-fused M -> G kernels; 1 matrix-core S1/S3 with a D x D workspace per slot of a chunk
  * for S3's output, the FP64 default, in FP32 only when the environment variable G4ACC_MTOG_E was
  * set at creation; 2 FP32 only, S3, the right site transform and the epilogue in one kernel with
- * that output kept in LDS, the FP32 default), "g4_sym" (1: transpose-symmetric G4 tiles where
- * the geometry allows them, the default there; 0: all tiles).
+ * that output kept in LDS, the FP32 default), "mtog_chunk" (slots per M -> G pass, below the
+ * number the workspace was sized for; 0: that number, a whole batch where its workspace fits
+ * 24 GiB), "g4_sym" (1: transpose-symmetric G4 tiles where the geometry allows them, the default
+ * there; 0: all tiles).
  * Defaults: mtog_mfma = 1 and, with max_walkers >= 8 and a measured channel set, g4_rank1 = 1 for
  * complex64 G on single-band 16-site clusters; 0 elsewhere.
  * Results do not depend on the tuning beyond FP32 summation order. */

@@ -476,9 +476,18 @@ def test_large_cluster_mtog_equals_fused(name, precision, monkeypatch):
     # in the transverse channel (both paths reach ~1e-6 there), so the bar is that the matrix-core
     # path is no less accurate than the register path it replaces.
     ref = g4acc.TpAccumulator(wl.geometry, precision="fp64", max_walkers=2, max_vertices=wl.n_max)
-    for acc in (large, reg, with_e, ref):
+    # One slot per M -> G pass: the chunk loop over a batch's slots gives the same G4 as one pass
+    # (both handles untuned, so the same kernels sum in the same order).
+    base = g4acc.TpAccumulator(wl.geometry, precision=precision, max_walkers=2, max_vertices=wl.n_max,
+                               tuned=False)
+    one = g4acc.TpAccumulator(wl.geometry, precision=precision, max_walkers=2, max_vertices=wl.n_max,
+                              tuned=False)
+    one.set_tuning(mtog_chunk=1)
+    for acc in (large, reg, with_e, ref, base, one):
         acc.resetAccumulation()
         acc.accumulate_batch(ms)
+    for x, o in zip(base.get_G4(), one.get_G4()):
+        assert linf(x, o) < 1e-13
     for x, y, z, r in zip(large.get_G4(), reg.get_G4(), with_e.get_G4(), ref.get_G4()):
         print(f"{name} {precision} G4 large {linf(r, x):.3e}, fused {linf(r, y):.3e}, "
               f"E workspace {linf(r, z):.3e} vs FP64", flush=True)
