@@ -496,18 +496,18 @@ __global__ void __launch_bounds__(256) mtog_s1_mfma(const SlotDesc* __restrict__
 #pragma unroll
     for (int t = 0; t < 4; ++t)
       cr[t] = ci[t] = V{0, 0, 0, 0};
+    // Every load is issued (clamped indices, finite data) and no load sits under a branch, which
+    // made each one wait on its own. The lane's M value (row j, step vertex i) is scaled by 0 past
+    // the site's vertices or the column count, which zeroes those steps' products; columns past
+    // nwE (clamped T reads) are computed but not stored.
     for (int i0 = b; i0 < e; i0 += 4) {
       const int ii = i0 + kl;
       const bool i_ok = ii < e;
       const int ic = i_ok ? ii : b;
-      Real m = Mz[jc + static_cast<size_t>(n_max) * ic];
-      if (!(i_ok && j_ok))
-        m = 0;
+      const Real m = Mz[jc + static_cast<size_t>(n_max) * ic] * static_cast<Real>(i_ok && j_ok ? 1 : 0);
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        C tt = Ts[wc[t] + static_cast<size_t>(nwE) * ic];
-        if (!(i_ok && w_ok[t]))
-          tt = C{0, 0};
+        const C tt = Ts[wc[t] + static_cast<size_t>(nwE) * ic];
         cr[t] = Mfma16<Real>::mma(m, tt.x, cr[t]);
         ci[t] = Mfma16<Real>::mma(m, tt.y, ci[t]);
       }
