@@ -347,11 +347,11 @@ def main():
 
     host = host_info()
     # CPU sample of the headline: one measurement per accumulator thread, as many threads as the
-    # box's CPU share and RAM allow (at most 16).
+    # host's CPU share and RAM allow (at most 16).
     from dca_amd import synth
     wl_h = synth.workload(args.config)
     per_acc = oracle_bytes_per_accumulator(wl_h.geometry)
-    # One command may use about 270 GiB of host memory on the GPU box: stay well inside it.
+    # The oracle accumulators take at most 200 GiB of host memory, 70 % of what is available.
     ram = min(host["mem_available_gb"] or 32, 200) * 2 ** 30
     threads = args.cpu_threads or max(1, min(16, host["nproc"], int(0.7 * ram // per_acc)))
     keep = 0 if args.no_cpu else threads
