@@ -490,39 +490,88 @@ __global__ void __launch_bounds__(256) mtog_s1_mfma(const SlotDesc* __restrict__
     w_ok[t] = w < nwE;
     wc[t] = w_ok[t] ? w : nwE - 1;
   }
-  for (int r = 0; r < NC; ++r) {
-    const int b = O[r], e = O[r + 1];
-    V cr[4], ci[4];
+  // The sites' steps form one stream (four vertices of one site per step), walked with two steps
+  // in flight: the loads of step i + 2 are issued while step i is multiplied. Every load is issued
+  // (clamped indices, finite data; past the stream's end a dummy step) and no load sits under a
+  // branch, so the wait before a step covers only that step's loads. The lane's M value (row j,
+  // step vertex i) is scaled by 0 past the site's vertices or the column count, which zeroes those
+  // products; columns past nwE (clamped T reads) are computed but not stored. A site's B rows are
+  // stored when the stream moves past it (zeros for sites without vertices).
+  __shared__ int Os[NC + 1];
+  if (threadIdx.x <= NC)
+    Os[threadIdx.x] = O[threadIdx.x];
+  __syncthreads();
+  int cs = NC, cj = 0, ce = 0;  // cursor (uniform): site of the next step, its first vertex, site end
+  auto open_site = [&](int from) {
+    cs = NC;
+    for (int r = from; r < NC; ++r)
+      if (Os[r] < Os[r + 1]) {
+        cs = r;
+        cj = Os[r];
+        ce = Os[r + 1];
+        break;
+      }
+  };
+  open_site(0);
+  Real mv[2];
+  C tv[2][4];
+  int sv[2];  // site of the step in the ring position (NC: none)
+  auto fetch = [&](int k) {
+    sv[k] = cs;
+    const int ii = cj + kl;
+    const bool i_ok = cs < NC && ii < ce;
+    const int ic = i_ok ? ii : 0;
+    mv[k] = Mz[jc + static_cast<size_t>(n_max) * ic] * static_cast<Real>(i_ok && j_ok ? 1 : 0);
 #pragma unroll
     for (int t = 0; t < 4; ++t)
-      cr[t] = ci[t] = V{0, 0, 0, 0};
-    // Every load is issued (clamped indices, finite data) and no load sits under a branch, which
-    // made each one wait on its own. The lane's M value (row j, step vertex i) is scaled by 0 past
-    // the site's vertices or the column count, which zeroes those steps' products; columns past
-    // nwE (clamped T reads) are computed but not stored.
-    for (int i0 = b; i0 < e; i0 += 4) {
-      const int ii = i0 + kl;
-      const bool i_ok = ii < e;
-      const int ic = i_ok ? ii : b;
-      const Real m = Mz[jc + static_cast<size_t>(n_max) * ic] * static_cast<Real>(i_ok && j_ok ? 1 : 0);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const C tt = Ts[wc[t] + static_cast<size_t>(nwE) * ic];
-        cr[t] = Mfma16<Real>::mma(m, tt.x, cr[t]);
-        ci[t] = Mfma16<Real>::mma(m, tt.y, ci[t]);
-      }
+      tv[k][t] = Ts[wc[t] + static_cast<size_t>(nwE) * ic];
+    if (cs < NC) {
+      cj += 4;
+      if (cj >= ce)
+        open_site(cs + 1);
     }
+  };
+  V cr[4], ci[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int jj = j0 + Mfma16<Real>::row(kl, i);
-      if (jj < n)
+  for (int t = 0; t < 4; ++t)
+    cr[t] = ci[t] = V{0, 0, 0, 0};
+  int cur = 0;  // site being accumulated
+  // Stores the accumulated site cur and zeros for the sites without vertices up to (excluding) s.
+  auto store_to = [&](int s) {
+    for (; cur < s; ++cur) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-          if (w_ok[t])
-            Bz[w0 + 16 * t + (lane & 15) + static_cast<size_t>(nwE) * (r + NC * static_cast<size_t>(jj))] =
-                C{cr[t][i], ci[t][i]};
+      for (int i = 0; i < 4; ++i) {
+        const int jj = j0 + Mfma16<Real>::row(kl, i);
+        if (jj < n)
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            if (w_ok[t])
+              Bz[w0 + 16 * t + (lane & 15) + static_cast<size_t>(nwE) * (cur + NC * static_cast<size_t>(jj))] =
+                  C{cr[t][i], ci[t][i]};
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        cr[t] = ci[t] = V{0, 0, 0, 0};
+    }
+  };
+  fetch(0);
+  fetch(1);
+  while (sv[0] < NC) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (sv[k] < NC) {
+        if (sv[k] != cur)
+          store_to(sv[k]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          cr[t] = Mfma16<Real>::mma(mv[k], tv[k][t].x, cr[t]);
+          ci[t] = Mfma16<Real>::mma(mv[k], tv[k][t].y, ci[t]);
+        }
+      }
+      fetch(k);
     }
   }
+  store_to(NC);
 }
 
 // Left half, step 2: the site transform A[(k, w)][jj] = sum_r exp(+i k.r) B[jj][r][w] for one
