@@ -428,7 +428,18 @@ __global__ void __launch_bounds__(256) mtog_permute_m(const SlotDesc* __restrict
   __shared__ Real tile[64][65];
   const int slot = slot0 + blockIdx.z;
   const SlotDesc d = desc[slot];
-  const int ii0 = blockIdx.x * 64, jj0 = blockIdx.y * 64;
+  // Workgroups are dealt to the 8 XCDs round-robin by id; a slot's tiles are remapped so that each
+  // XCD takes a contiguous range (column blocks slowest), and the tiles that gather from the same 64
+  // columns of M, whose cache lines they share, run on one XCD.
+  unsigned bx = blockIdx.x, by = blockIdx.y;
+  const unsigned per_slot = gridDim.x * gridDim.y;
+  if (per_slot % 8 == 0) {
+    const unsigned lin = blockIdx.x + gridDim.x * blockIdx.y;
+    const unsigned lid = (lin % 8) * (per_slot / 8) + lin / 8;
+    bx = lid % gridDim.x;
+    by = lid / gridDim.x;
+  }
+  const int ii0 = static_cast<int>(bx) * 64, jj0 = static_cast<int>(by) * 64;
   if (ii0 >= d.n || jj0 >= d.n)
     return;
   const int* P = perm + static_cast<size_t>(slot) * n_max;
