@@ -150,7 +150,12 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
   // two-walker pipeline (below): each walker's terms go straight into the FP64 totals, whose
   // registers would otherwise hold FP32 partials.
   constexpr bool kPipe = RX && TT == 1024 && PPT == 2;
-  constexpr bool kChunk = sizeof(Real) == 4 && !kPipe;
+  // 1024-thread tiles of four pairs per thread (8x8) with complex64 G: every prefetch load is
+  // issued (clamped indices and walker) and the thread's X entries of the current walker are
+  // loaded before the next walker's prefetch, so the products wait for X only. Each walker's
+  // terms go straight into the FP64 totals, whose registers would otherwise hold FP32 partials.
+  constexpr bool kXFirst = !RX && TT == 1024 && PPT == 4 && sizeof(Real) == 4;
+  constexpr bool kChunk = sizeof(Real) == 4 && !kPipe && !kXFirst;
   constexpr int FLUSH = 8;
   constexpr int PC = kChunk ? PPT : 1, QC = kChunk ? QG : 1;
   // Registers: with many pairs per thread (or 1024-thread workgroups, 128 VGPRs) X is read in place.
@@ -274,6 +279,27 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
   // rank-one factors and (for few pairs per thread) the thread's own X entries.
   C pre[4][PPT], pre_uv, pre_x[2][kPrefetchX ? PPT : 1];
   auto prefetch = [&](int m) {
+    if constexpr (kXFirst) {
+      const int mc = m < a.n_walkers ? m : a.n_walkers - 1;
+      const C* H0 = a.G + (2 * static_cast<size_t>(mc)) * DD;
+      const C* H1 = H0 + DD;
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        const int idx = threadIdx.x + TT * j;
+        const int idc = idx < nc * nc ? idx : 0;
+        const size_t go = (idc % nc) + D * (idc / nc);
+        if (needY) {
+          pre[0][j] = H0[yoff + go];
+          pre[1][j] = H1[yoff + go];
+        }
+        if (needZ) {
+          pre[2][j] = H0[zoff + go];
+          pre[3][j] = H1[zoff + go];
+        }
+      }
+      pre_uv = a.UV[static_cast<size_t>(mc) * 4 * plane + (u_valid ? u_src : 0)];
+      return;
+    }
     const C* H0 = a.G + (2 * static_cast<size_t>(m)) * DD;
     const C* H1 = H0 + DD;
 #pragma unroll
@@ -329,14 +355,29 @@ __global__ void __launch_bounds__(TT) g4_tiled_kernel(TiledArgs<Real> a) {
     __syncthreads();
     const Real alpha = static_cast<Real>(0.5 * a.desc[2 * m].sign);
     const C* G0m = a.G + (2 * static_cast<size_t>(m)) * DD;
-    if (m + 1 < a.n_walkers)
+    C xr[kXFirst ? PPT : 1][2];
+    if constexpr (kXFirst) {
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        const size_t xo = xoff + k1[j] + D * k2[j];  // pair 0 for idle pairs
+        xr[j][0] = G0m[xo];
+        xr[j][1] = G0m[DD + xo];
+      }
       prefetch(m + 1);
+    }
+    else if (m + 1 < a.n_walkers) {
+      prefetch(m + 1);
+    }
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
       if (!ok[j])
         continue;
       C x0, x1;
-      if constexpr (kPrefetchX) {
+      if constexpr (kXFirst) {
+        x0 = xr[j][0];
+        x1 = xr[j][1];
+      }
+      else if constexpr (kPrefetchX) {
         x0 = xw[0][j];
         x1 = xw[1][j];
       }
